@@ -1,0 +1,35 @@
+"""Extract H/Li/C/N/O subsets of the basis-set-exchange JSON files that
+qdk-chemistry ships (cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/
+compressed/<name>.tar.gz, member basis/<name>.json) into data/basis/.
+
+Only `electron_shells` of the kept elements are copied; the values are data
+(exponents and contraction coefficients as strings, untouched).  Run once in a
+container that holds the reference checkout; the output JSON is committed.
+"""
+import io
+import json
+import os
+import sys
+import tarfile
+
+REF = "/root/reference/cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/compressed"
+NAMES = ["cc-pvdz", "cc-pvtz", "def2-svp", "def2-tzvp", "6-31g_st_", "sto-3g"]
+ELEMENTS = ["1", "3", "6", "7", "8"]
+
+
+def main(out_dir):
+    os.makedirs(out_dir, exist_ok=True)
+    for name in NAMES:
+        with tarfile.open(os.path.join(REF, name + ".tar.gz")) as tf:
+            member = tf.getmember(f"basis/{name}.json")
+            data = json.load(io.TextIOWrapper(tf.extractfile(member)))
+        keep = {"name": data.get("name", name), "elements": {}}
+        for z in ELEMENTS:
+            if z in data["elements"]:
+                keep["elements"][z] = {"electron_shells": data["elements"][z]["electron_shells"]}
+        with open(os.path.join(out_dir, name + ".json"), "w") as f:
+            json.dump(keep, f, indent=0)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(__file__), "basis"))
