@@ -1,0 +1,118 @@
+/*
+ * qfe.h -- C ABI of the MI355X Fock-build engine (ERIs + Schwarz screening +
+ * J/K digestion) for qdk-chemistry's SCF solver.
+ *
+ * It is the drop-in for the reference's integral-direct J/K engine
+ * (cpp/src/qdk/chemistry/algorithms/microsoft/scf/src/eri/LIBINT2_DIRECT/
+ * libint2_direct.cpp) behind the abstract qdk::chemistry::scf::ERI interface
+ * (scf/include/qdk/chemistry/scf/core/eri.h:23-176), in the slot the reference
+ * reserves for a GPU engine (ERIMethod::Rys, core/enums.h:22-35, created in
+ * scf/src/eri/eri.cpp:36-68).  Plain pointers and sizes only.
+ *
+ * Conventions (match the reference):
+ *   - AO order is the order of the shells passed in qfe_basis (the reference
+ *     sorts shells by L, then descending leading exponent), functions within
+ *     a shell: cartesian CCA order for l < 2, solid harmonics m = -l..l for
+ *     l >= 2 (pure bases).
+ *   - Contraction coefficients are used as given (the reference builds its
+ *     libint2 shells with embed_normalization=false, libint2_util.cpp:15-28).
+ *   - Matrices are row-major double, ndm x nbf x nbf.
+ *   - J and K come back symmetrised; K includes the (alpha + beta) scale.
+ *   - Gradient outputs are 3*natoms, xyz-major: index = atom + xyz*natoms
+ *     (libint2_direct.cpp:824).
+ * Every function returns 0 on success and a non-zero status otherwise; the
+ * message is available from qfe_last_error() (thread-local).
+ */
+#ifndef QFE_H
+#define QFE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define QFE_OK 0
+#define QFE_ERR_INVALID 1     /* invalid argument (std::invalid_argument in the reference) */
+#define QFE_ERR_RUNTIME 2     /* runtime failure (std::runtime_error), e.g. NaN density */
+#define QFE_ERR_DEVICE 3      /* HIP runtime error */
+#define QFE_ERR_UNSUPPORTED 4 /* feature outside this engine (e.g. range-separated exchange) */
+
+typedef struct qfe_handle qfe_handle;
+
+/* Basis set in AO order.  Replaces qdk::chemistry::scf::BasisSet
+ * (scf/include/qdk/chemistry/scf/core/basis_set.h:30-58) at the boundary. */
+typedef struct {
+  int nshells;
+  const int* l;            /* [nshells] angular momentum */
+  const int* atom;         /* [nshells] atom index */
+  const double* centers;   /* [nshells][3] bohr */
+  const int* prim_offset;  /* [nshells + 1] offsets into exps/coefs */
+  const double* exps;      /* primitive exponents */
+  const double* coefs;     /* contraction coefficients (normalisation included) */
+  int pure;                /* 1: solid harmonics for l >= 2 */
+  int natoms;
+  const double* charges;   /* [natoms] nuclear charges (one-electron integrals) */
+  const double* coords;    /* [natoms][3] bohr */
+} qfe_basis;
+
+/* Engine settings.  Field names follow ERIConfig (core/scf.h:65-94). */
+typedef struct {
+  double eri_threshold;        /* Schwarz x density screening threshold (1e-12 plugin default) */
+  double shell_pair_threshold; /* overlap threshold for significant pairs (1e-12) */
+  int device_id;               /* HIP device ordinal */
+  int mix_precision;           /* 1: FP32 for quartets whose Schwarz bound < mix_threshold */
+  double mix_threshold;        /* 1e-6 */
+  int verbose;
+} qfe_config;
+
+/* Per-build statistics (the last build). */
+typedef struct {
+  long long quartets;          /* shell quartets that passed screening */
+  long long quartets_fp32;     /* of which evaluated in FP32 */
+  long long prim_quartets;     /* primitive quartets of the evaluated shell quartets */
+  long long func_quartets;     /* function quartets */
+  double flops_fp64;           /* algorithmic FP64 flops (see DESIGN) */
+  double flops_fp32;
+  double ms_screen;            /* task-list construction */
+  double ms_eri;               /* ERI + digestion kernels */
+  double ms_total;             /* whole build, device time */
+  long long significant_pairs;
+} qfe_stats;
+
+/* Create the engine: significant pairs, primitive-pair tables, Schwarz.
+ * Replaces libint2::direct::ERI::ERI (libint2_direct.cpp:204-226). */
+int qfe_create(const qfe_basis* basis, const qfe_config* cfg, qfe_handle** out);
+
+/* J and K for ndm host densities.  Replaces ERI::build_JK
+ * (eri.h:79-80; libint2_direct.cpp:256-626).  omega != 0 is unsupported. */
+int qfe_build_jk(qfe_handle* h, const double* P, double* J, double* K, double alpha, double beta,
+                 double omega, int ndm);
+
+/* Same on device-resident buffers (HIP device pointers, nbf x nbf each);
+ * stream may be NULL for the engine's own stream.  J or K may be NULL. */
+int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK, double alpha,
+                        double beta, double omega, void* stream);
+
+/* ERI nuclear-gradient contributions dJ, dK (3*natoms, xyz-major) for one
+ * density.  Replaces ERI::get_gradients (eri.h:100-101;
+ * libint2_direct.cpp:646-890). */
+int qfe_gradients(qfe_handle* h, const double* P, double* dJ, double* dK, double alpha,
+                  double beta, double omega);
+
+/* One-electron matrices (host): op 0 = overlap, 1 = kinetic, 2 = nuclear
+ * attraction.  Harness support for the SCF driver (scf/src/util/int1e.cpp). */
+int qfe_one_body(const qfe_basis* basis, int op, double* out);
+
+/* Gradient of sum_mn W_mn d(O)_mn / dR for op 0/1/2 (3*natoms, atom-major). */
+int qfe_one_body_grad(const qfe_basis* basis, int op, const double* W, double* grad);
+
+int qfe_get_stats(const qfe_handle* h, qfe_stats* out);
+int qfe_nbf(const qfe_handle* h);
+void qfe_destroy(qfe_handle* h);
+const char* qfe_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QFE_H */

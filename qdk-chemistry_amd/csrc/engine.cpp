@@ -1,0 +1,617 @@
+// Host side of the Fock-build engine: the C ABI of include/qfe.h.
+//
+// qfe_create   restates libint2::direct::ERI::ERI (libint2_direct.cpp:204-226):
+//              significant shell pairs (same centre or ||S_ab||_F >=
+//              shell_pair_threshold, :83-151), primitive-pair tables, Schwarz
+//              factors on the GPU (schwarz.cpp:26-78); then buckets the pairs
+//              by (class, primitive count), Schwarz-descending inside a bucket.
+// qfe_build_jk restates ERI::build_JK (:256-626) as: shell-block density
+//              norms -> two-pass screened task list (same test as :394-402,
+//              same canonical quartet set and degeneracy as :357-411) ->
+//              class kernels with J/K digestion -> symmetrisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <dlfcn.h>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qfe.h"
+#include "eri_core.hpp"
+#include "host_common.hpp"
+#include "kernels.hpp"
+
+namespace qfe {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(x)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      throw DeviceError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);  \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) count = 1;
+    HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+};
+
+// ------------------------------------------------------------ Rys table
+static std::string library_dir() {
+  Dl_info info;
+  if (dladdr((void*)&library_dir, &info) && info.dli_fname) {
+    std::string f = info.dli_fname;
+    auto k = f.rfind('/');
+    if (k != std::string::npos) return f.substr(0, k);
+  }
+  return ".";
+}
+
+struct RysTableHost {
+  int nmax = 0, deg = 0;
+  double h = 1.0;
+  std::vector<std::vector<double>> coef;  // per n
+  std::vector<int> nint;
+  std::vector<double> tmax;
+  std::vector<std::vector<double>> hx2, hw;
+};
+
+static RysTableHost load_rys_table() {
+  RysTableHost t;
+  const char* env = getenv("QFE_RYS_TABLE");
+  std::string path = env ? env : library_dir() + "/rys_table.bin";
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open Rys table " + path);
+  int32_t nmax, deg;
+  double h;
+  f.read((char*)&nmax, 4);
+  f.read((char*)&deg, 4);
+  f.read((char*)&h, 8);
+  if (!f || deg != RYS_DEG || nmax < 1 || nmax > 8) throw std::runtime_error("bad Rys table header in " + path);
+  t.nmax = nmax;
+  t.deg = deg;
+  t.h = h;
+  t.coef.resize(nmax + 1);
+  t.nint.assign(nmax + 1, 0);
+  t.tmax.assign(nmax + 1, 0.0);
+  t.hx2.resize(nmax + 1);
+  t.hw.resize(nmax + 1);
+  for (int n = 1; n <= nmax; ++n) {
+    int32_t nint;
+    double tmax;
+    f.read((char*)&nint, 4);
+    f.read((char*)&tmax, 8);
+    if (!f || nint < 1 || nint > 4096) throw std::runtime_error("bad Rys table in " + path);
+    t.nint[n] = nint;
+    t.tmax[n] = tmax;
+    t.coef[n].resize((size_t)nint * 2 * n * (deg + 1));
+    f.read((char*)t.coef[n].data(), t.coef[n].size() * 8);
+    t.hx2[n].resize(n);
+    t.hw[n].resize(n);
+    f.read((char*)t.hx2[n].data(), n * 8);
+    f.read((char*)t.hw[n].data(), n * 8);
+  }
+  if (!f) throw std::runtime_error("truncated Rys table " + path);
+  return t;
+}
+
+// loaded once, thread-safe (function-local static)
+static const RysTableHost& rys_table() {
+  static const RysTableHost t = load_rys_table();
+  return t;
+}
+
+// host evaluation (used by the one-electron code)
+void rys_host(int n, double T, double* u, double* w) {
+  const RysTableHost& t = rys_table();
+  if (T >= t.tmax[n]) {
+    const double it = 1.0 / T, is = std::sqrt(it);
+    for (int i = 0; i < n; ++i) {
+      u[i] = t.hx2[n][i] * it;
+      w[i] = t.hw[n][i] * is;
+    }
+    return;
+  }
+  const int k = (int)T;
+  const double x = T - (k + 0.5);
+  const double* c = t.coef[n].data() + (size_t)k * 2 * n * (t.deg + 1);
+  for (int i = 0; i < 2 * n; ++i) {
+    const double* ci = c + i * (t.deg + 1);
+    double v = ci[t.deg];
+    for (int d = t.deg - 1; d >= 0; --d) v = v * x + ci[d];
+    if (i < n)
+      u[i] = v;
+    else
+      w[i - n] = v;
+  }
+}
+
+// ------------------------------------------------------------ the engine
+struct Engine {
+  // input
+  BasisHost basis;
+  double eri_thr = 1e-12, sp_thr = 1e-12;
+  int device = 0;
+  bool mix = false;
+  double mix_thr = 1e-6;
+  int verbose = 0;
+  int nbf = 0, nsh = 0, natoms = 0;
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+
+  // pairs (sorted by class, nprim, Q desc)
+  std::vector<PairHdr> pairs_h;
+  std::vector<double> Q_h;
+  std::vector<int2> buckets_h;       // [begin, end) in the sorted pair array
+  std::vector<int> bucket_cls_h;
+  std::vector<int> pair_bucket_h;
+  std::vector<int2> items_h;         // (bra pair, ket bucket)
+  std::vector<int> cp_item_begin;    // per class pair: first item (size ncp + 1)
+  std::vector<int> cp_bra, cp_ket;   // class pair -> (bra class, ket class)
+  long long prim_pairs = 0;
+
+  DevBuf<PairHdr> pairs_d;
+  DevBuf<double4> ppos_d;
+  DevBuf<double> pk_d;
+  DevBuf<double> Q_d;
+  DevBuf<int2> buckets_d;
+  DevBuf<int> pair_bucket_d;
+  DevBuf<int2> items_d;
+  DevBuf<int> counts_d;
+  DevBuf<long long> offsets_d;
+  DevBuf<long long> bidx_d, bval_d;
+  DevBuf<unsigned char> scan_tmp_d;
+  size_t scan_tmp_bytes = 0;
+  DevBuf<int2> tasks_d;
+  DevBuf<int> sh_bf0_d, sh_nf_d;
+  DevBuf<double> Pn_d, pmax_d;
+  DevBuf<int> bad_d;
+  DevBuf<double> P_d, J_d, K_d;
+  std::vector<DevBuf<double>> rys_d;
+  RysArgs ra{};
+
+  qfe_stats stats{};
+
+  ~Engine() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void setup_rys() {
+    const RysTableHost& t = rys_table();
+    rys_d.resize(t.nmax + 1);
+    std::memset(&ra, 0, sizeof(ra));
+    for (int n = 1; n <= t.nmax && n <= 8; ++n) {
+      rys_d[n].upload(t.coef[n], stream);
+      ra.coef[n] = rys_d[n].p;
+      ra.tmax[n] = t.tmax[n];
+      for (int i = 0; i < n; ++i) {
+        ra.hx2[n][i] = t.hx2[n][i];
+        ra.hw[n][i] = t.hw[n][i];
+      }
+    }
+  }
+
+  void build_pairs() {
+    // significant pairs (libint2_direct.cpp:83-151)
+    struct RawPair {
+      int s1, s2;  // oriented: l(s1) >= l(s2)
+    };
+    std::vector<RawPair> raw;
+    for (int s1 = 0; s1 < nsh; ++s1)
+      for (int s2 = 0; s2 <= s1; ++s2) {
+        const auto& A = basis.shells[s1];
+        const auto& B = basis.shells[s2];
+        bool same = A.O[0] == B.O[0] && A.O[1] == B.O[1] && A.O[2] == B.O[2];
+        bool sig = same;
+        if (!same) sig = overlap_block_norm(basis, s1, s2) >= sp_thr;
+        if (!sig) continue;
+        if (A.l >= B.l)
+          raw.push_back({s1, s2});
+        else
+          raw.push_back({s2, s1});
+      }
+    // primitive pairs
+    std::vector<PairHdr> hdr(raw.size());
+    std::vector<double4> ppos;
+    std::vector<double> pk;
+    for (size_t i = 0; i < raw.size(); ++i) {
+      const auto& A = basis.shells[raw[i].s1];
+      const auto& B = basis.shells[raw[i].s2];
+      PairHdr& h = hdr[i];
+      for (int d = 0; d < 3; ++d) {
+        h.A[d] = A.O[d];
+        h.B[d] = B.O[d];
+      }
+      h.sa = raw[i].s1;
+      h.sb = raw[i].s2;
+      h.bfa = basis.bf0[h.sa];
+      h.bfb = basis.bf0[h.sb];
+      h.cls = A.l * (A.l + 1) / 2 + B.l;
+      h.prim0 = (int)ppos.size();
+      const double AB2 = (A.O[0] - B.O[0]) * (A.O[0] - B.O[0]) + (A.O[1] - B.O[1]) * (A.O[1] - B.O[1]) +
+                         (A.O[2] - B.O[2]) * (A.O[2] - B.O[2]);
+      for (size_t ia = 0; ia < A.a.size(); ++ia)
+        for (size_t ib = 0; ib < B.a.size(); ++ib) {
+          const double a = A.a[ia], b = B.a[ib], p = a + b;
+          const double K = A.c[ia] * B.c[ib] * std::exp(-a * b / p * AB2);
+          // primitive pairs below 1e-26 cannot contribute at double precision
+          if (std::fabs(K) < 1e-26) continue;
+          ppos.push_back(make_double4(p, (a * A.O[0] + b * B.O[0]) / p, (a * A.O[1] + b * B.O[1]) / p,
+                                      (a * A.O[2] + b * B.O[2]) / p));
+          pk.push_back(K / p);
+        }
+      h.nprim = (int)ppos.size() - h.prim0;
+      h.pad = 0;
+    }
+    prim_pairs = (long long)ppos.size();
+    const int npair = (int)hdr.size();
+    stats.significant_pairs = npair;
+
+    // Schwarz on the GPU, per pair class
+    pairs_d.upload(hdr, stream);
+    ppos_d.upload(ppos, stream);
+    pk_d.upload(pk, stream);
+    DevBuf<double> q2;
+    q2.alloc(npair);
+    HIP_CHECK(hipMemsetAsync(q2.p, 0, sizeof(double) * npair, stream));
+    const int ncls = (QFE_MAX_L + 1) * (QFE_MAX_L + 2) / 2;
+    std::vector<std::vector<int>> bycls(ncls);
+    for (int i = 0; i < npair; ++i) bycls[hdr[i].cls].push_back(i);
+    std::vector<DevBuf<int>> plist(ncls);
+    for (int c = 0; c < ncls; ++c) {
+      if (bycls[c].empty()) continue;
+      plist[c].upload(bycls[c], stream);
+      SchwarzLaunch a{plist[c].p, (int)bycls[c].size(), pairs_d.p, ppos_d.p, pk_d.p, &ra, q2.p};
+      launch_schwarz(c, basis.pure, a, stream);
+      HIP_CHECK(hipGetLastError());
+    }
+    std::vector<double> q2h(npair);
+    HIP_CHECK(hipMemcpyAsync(q2h.data(), q2.p, sizeof(double) * npair, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    std::vector<double> Q(npair);
+    for (int i = 0; i < npair; ++i) Q[i] = std::sqrt(std::sqrt(q2h[i]));
+
+    // sort by (class, nprim, Q desc)
+    std::vector<int> ord(npair);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+      if (hdr[x].cls != hdr[y].cls) return hdr[x].cls < hdr[y].cls;
+      if (hdr[x].nprim != hdr[y].nprim) return hdr[x].nprim < hdr[y].nprim;
+      return Q[x] > Q[y];
+    });
+    pairs_h.resize(npair);
+    Q_h.resize(npair);
+    std::vector<double4> ppos2;
+    std::vector<double> pk2;
+    ppos2.reserve(ppos.size());
+    pk2.reserve(pk.size());
+    for (int i = 0; i < npair; ++i) {
+      PairHdr h = hdr[ord[i]];
+      const int old0 = h.prim0;
+      h.prim0 = (int)ppos2.size();
+      for (int k = 0; k < h.nprim; ++k) {
+        ppos2.push_back(ppos[old0 + k]);
+        pk2.push_back(pk[old0 + k]);
+      }
+      pairs_h[i] = h;
+      Q_h[i] = Q[ord[i]];
+    }
+    // buckets
+    buckets_h.clear();
+    bucket_cls_h.clear();
+    pair_bucket_h.assign(npair, 0);
+    for (int i = 0; i < npair;) {
+      int j = i;
+      while (j < npair && pairs_h[j].cls == pairs_h[i].cls && pairs_h[j].nprim == pairs_h[i].nprim) ++j;
+      for (int k = i; k < j; ++k) pair_bucket_h[k] = (int)buckets_h.size();
+      buckets_h.push_back(make_int2(i, j));
+      bucket_cls_h.push_back(pairs_h[i].cls);
+      i = j;
+    }
+    // work items ordered by class pair, then (bra bucket, ket bucket, bra pair)
+    const int nb = (int)buckets_h.size();
+    const int ncp = ncls * (ncls + 1) / 2;
+    cp_bra.assign(ncp, 0);
+    cp_ket.assign(ncp, 0);
+    for (int b = 0; b < ncls; ++b)
+      for (int k = 0; k <= b; ++k) {
+        cp_bra[b * (b + 1) / 2 + k] = b;
+        cp_ket[b * (b + 1) / 2 + k] = k;
+      }
+    std::vector<std::vector<int2>> per_cp(ncp);
+    for (int bb = 0; bb < nb; ++bb)
+      for (int kb = 0; kb <= bb; ++kb) {
+        const int cb = bucket_cls_h[bb], ck = bucket_cls_h[kb];
+        const int cp = cb * (cb + 1) / 2 + ck;  // cb >= ck since buckets are class-sorted
+        for (int p = buckets_h[bb].x; p < buckets_h[bb].y; ++p) per_cp[cp].push_back(make_int2(p, kb));
+      }
+    items_h.clear();
+    cp_item_begin.assign(ncp + 1, 0);
+    for (int cp = 0; cp < ncp; ++cp) {
+      cp_item_begin[cp] = (int)items_h.size();
+      items_h.insert(items_h.end(), per_cp[cp].begin(), per_cp[cp].end());
+    }
+    cp_item_begin[ncp] = (int)items_h.size();
+
+    pairs_d.upload(pairs_h, stream);
+    ppos_d.upload(ppos2, stream);
+    pk_d.upload(pk2, stream);
+    Q_d.upload(Q_h, stream);
+    buckets_d.upload(buckets_h, stream);
+    pair_bucket_d.upload(pair_bucket_h, stream);
+    items_d.upload(items_h, stream);
+    const long long nitems = (long long)items_h.size();
+    counts_d.alloc(nitems + 1);
+    HIP_CHECK(hipMemsetAsync(counts_d.p, 0, sizeof(int) * (nitems + 1), stream));
+    offsets_d.alloc(nitems + 1);
+    scan_tmp_bytes = scan_temp_bytes(nitems + 1);
+    scan_tmp_d.alloc(scan_tmp_bytes);
+    std::vector<long long> bidx(ncp + 1);
+    for (int cp = 0; cp <= ncp; ++cp) bidx[cp] = cp_item_begin[cp];
+    bidx_d.upload(bidx, stream);
+    bval_d.alloc(ncp + 1);
+    HIP_CHECK(hipStreamSynchronize(stream));
+    if (verbose)
+      fprintf(stderr, "[qfe] nsh=%d nbf=%d pairs=%d prim_pairs=%lld buckets=%d items=%lld\n", nsh, nbf,
+              npair, prim_pairs, nb, nitems);
+  }
+
+  void create(const qfe_basis* b, const qfe_config* cfg) {
+    basis = BasisHost::from_c(b);
+    nsh = basis.nsh;
+    nbf = basis.nbf;
+    natoms = basis.natoms;
+    if (nsh <= 0) throw std::invalid_argument("empty basis");
+    if (!basis.pure) throw Unsupported("cartesian (non-pure) bases are not compiled into this engine");
+    for (const auto& s : basis.shells)
+      if (s.l > QFE_MAX_L)
+        throw Unsupported("shell angular momentum " + std::to_string(s.l) + " exceeds the compiled maximum " +
+                          std::to_string(QFE_MAX_L));
+    if (cfg) {
+      eri_thr = cfg->eri_threshold;
+      sp_thr = cfg->shell_pair_threshold;
+      device = cfg->device_id;
+      mix = cfg->mix_precision != 0;
+      mix_thr = cfg->mix_threshold;
+      verbose = cfg->verbose;
+    }
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw std::invalid_argument("device_id out of range");
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    setup_rys();
+    std::vector<int> bf0(nsh), nf(nsh);
+    for (int s = 0; s < nsh; ++s) {
+      bf0[s] = basis.bf0[s];
+      nf[s] = basis.bf0[s + 1] - basis.bf0[s];
+    }
+    sh_bf0_d.upload(bf0, stream);
+    sh_nf_d.upload(nf, stream);
+    Pn_d.alloc((size_t)nsh * nsh);
+    pmax_d.alloc(1);
+    bad_d.alloc(1);
+    build_pairs();
+  }
+
+  // J/K for one density on device buffers
+  void build_device(const double* dP, double* dJ, double* dK, double kscale, hipStream_t s) {
+    HIP_CHECK(hipSetDevice(device));
+    const size_t nn = (size_t)nbf * nbf;
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    HIP_CHECK(hipMemsetAsync(bad_d.p, 0, sizeof(int), s));
+    launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, bad_d.p, pmax_d.p, s);
+    HIP_CHECK(hipGetLastError());
+    const long long nitems = (long long)items_h.size();
+    launch_task_count(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, pairs_d.p, Q_d.p, Pn_d.p, nsh,
+                      pmax_d.p, eri_thr, counts_d.p, s);
+    HIP_CHECK(hipGetLastError());
+    launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, nitems + 1, s);
+    HIP_CHECK(hipGetLastError());
+    const int ncp = (int)cp_bra.size();
+    launch_gather_offsets(offsets_d.p, bidx_d.p, ncp + 1, bval_d.p, s);
+    std::vector<long long> bval(ncp + 1);
+    int bad = 0;
+    HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * (ncp + 1), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&bad, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (bad) throw std::runtime_error("Density matrix contains NaN/Inf values.");
+    const long long total = bval[ncp];
+    if (total > 0) tasks_d.alloc((size_t)total);
+    launch_task_write(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, pairs_d.p, Q_d.p, Pn_d.p, nsh,
+                      pmax_d.p, eri_thr, offsets_d.p, tasks_d.p, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    if (dJ) HIP_CHECK(hipMemsetAsync(dJ, 0, nn * sizeof(double), s));
+    if (dK) HIP_CHECK(hipMemsetAsync(dK, 0, nn * sizeof(double), s));
+    // J and K buffers are both required by the digestion kernels
+    double* Jw = dJ;
+    double* Kw = dK;
+    if (!Jw) {
+      J_d.alloc(nn);
+      Jw = J_d.p;
+      HIP_CHECK(hipMemsetAsync(Jw, 0, nn * sizeof(double), s));
+    }
+    if (!Kw) {
+      K_d.alloc(nn);
+      Kw = K_d.p;
+      HIP_CHECK(hipMemsetAsync(Kw, 0, nn * sizeof(double), s));
+    }
+    long long nq = 0;
+    for (int cp = 0; cp < ncp; ++cp) {
+      const long long n = bval[cp + 1] - bval[cp];
+      if (n <= 0) continue;
+      nq += n;
+      JKLaunch a{tasks_d.p + bval[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jw, Kw, nbf, kscale};
+      launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipEventRecord(ev[2], s));
+    launch_symmetrize(dJ, dK, nbf, s);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ev[3], s));
+    HIP_CHECK(hipEventSynchronize(ev[3]));
+    float t01 = 0, t12 = 0, t03 = 0;
+    HIP_CHECK(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&t03, ev[0], ev[3]));
+    stats.quartets = nq;
+    stats.quartets_fp32 = 0;
+    stats.ms_screen = t01;
+    stats.ms_eri = t12;
+    stats.ms_total = t03;
+  }
+
+  void build_host(const double* P, double* J, double* K, double alpha, double beta, double omega, int ndm) {
+    if (std::fabs(omega) > 1e-12)
+      throw Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
+    if (ndm < 1 || ndm > 2) throw std::invalid_argument("ndm must be 1 or 2");
+    const size_t nn = (size_t)nbf * nbf;
+    P_d.alloc(nn);
+    J_d.alloc(nn);
+    K_d.alloc(nn);
+    for (int idm = 0; idm < ndm; ++idm) {
+      HIP_CHECK(hipMemcpyAsync(P_d.p, P + idm * nn, nn * sizeof(double), hipMemcpyHostToDevice, stream));
+      build_device(P_d.p, J_d.p, K_d.p, alpha + beta, stream);
+      if (J) HIP_CHECK(hipMemcpyAsync(J + idm * nn, J_d.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (K) HIP_CHECK(hipMemcpyAsync(K + idm * nn, K_d.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+    }
+  }
+};
+
+}  // namespace qfe
+
+// ------------------------------------------------------------ C ABI
+struct qfe_handle {
+  qfe::Engine e;
+};
+
+using qfe::g_last_error;
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return QFE_OK;
+  } catch (const std::invalid_argument& ex) {
+    g_last_error = ex.what();
+    return QFE_ERR_INVALID;
+  } catch (const qfe::Unsupported& ex) {
+    g_last_error = ex.what();
+    return QFE_ERR_UNSUPPORTED;
+  } catch (const qfe::DeviceError& ex) {
+    g_last_error = ex.what();
+    return QFE_ERR_DEVICE;
+  } catch (const std::exception& ex) {
+    g_last_error = ex.what();
+    return QFE_ERR_RUNTIME;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return QFE_ERR_RUNTIME;
+  }
+}
+
+extern "C" {
+
+int qfe_create(const qfe_basis* basis, const qfe_config* cfg, qfe_handle** out) {
+  return guarded([&] {
+    if (!basis || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    auto h = std::make_unique<qfe_handle>();
+    h->e.create(basis, cfg);
+    *out = h.release();
+  });
+}
+
+int qfe_build_jk(qfe_handle* h, const double* P, double* J, double* K, double alpha, double beta,
+                 double omega, int ndm) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    if (!P) throw std::invalid_argument("null density");
+    h->e.build_host(P, J, K, alpha, beta, omega, ndm);
+  });
+}
+
+int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK, double alpha, double beta,
+                        double omega, void* stream) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    if (!dP) throw std::invalid_argument("null density");
+    if (std::fabs(omega) > 1e-12)
+      throw qfe::Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
+    h->e.build_device(dP, dJ, dK, alpha + beta, stream ? (hipStream_t)stream : h->e.stream);
+  });
+}
+
+int qfe_gradients(qfe_handle* h, const double* P, double* dJ, double* dK, double alpha, double beta,
+                  double omega) {
+  return guarded([&] {
+    (void)h;
+    (void)P;
+    (void)dJ;
+    (void)dK;
+    (void)alpha;
+    (void)beta;
+    (void)omega;
+    throw qfe::Unsupported("nuclear gradients are not available yet");
+  });
+}
+
+int qfe_get_stats(const qfe_handle* h, qfe_stats* out) {
+  return guarded([&] {
+    if (!h || !out) throw std::invalid_argument("null argument");
+    *out = h->e.stats;
+  });
+}
+
+int qfe_nbf(const qfe_handle* h) { return h ? h->e.nbf : -1; }
+
+void qfe_destroy(qfe_handle* h) { delete h; }
+
+const char* qfe_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+// Per-class ERI kernels (templates), instantiated one class per translation
+// unit by the build (gen/cls_*.hip) to keep compile times bounded.
+//   k_eri_jk   Rys ERIs of one class (one bra chunk) + J/K digestion
+//              (libint2_direct.cpp:414-567)
+//   k_schwarz  sum of squares of (ab|ab) for Q_ab = sqrt ||(ab|ab)||_F
+//              (schwarz.cpp:26-78)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "eri_core.hpp"
+#include "kernels.hpp"
+
+namespace qfe {
+
+// ------------------------------------------------------------ host sfor
+template <typename F, int... Is>
+inline void hsfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+inline void hsfor(F&& f) {
+  hsfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+constexpr int kAccMax = 48;  // register budget (doubles) for one thread's accumulators
+template <class C>
+constexpr int chunk_size() {
+  int ch = kAccMax / C::NKETC;
+  if (ch < 1) ch = 1;
+  if (ch > C::NBRA) ch = C::NBRA;
+  return ch;
+}
+
+// ------------------------------------------------------------ J/K kernel
+template <class C, int Q0, int Q1>
+__global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, long long ntask,
+                                                const PairHdr* __restrict__ pairs,
+                                                const double4* __restrict__ ppos,
+                                                const double* __restrict__ pk, RysArgs ra,
+                                                const double* __restrict__ P, double* __restrict__ J,
+                                                double* __restrict__ K, int nbf, double kscale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntask) return;
+  const int2 tk = tasks[t];
+  const PairHdr bra = pairs[tk.x];
+  const PairHdr ket = pairs[tk.y];
+  constexpr int NQ = Q1 - Q0;
+  double acc[NQ][C::NKETC];
+  contract_chunk<C, Q0, Q1>(bra, ket, ppos, pk, ra, acc);
+
+  const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
+                     (tk.x == tk.y ? 1.0 : 2.0);
+  const int i0 = bra.bfa, j0 = bra.bfb, k0 = ket.bfa, l0 = ket.bfb;
+  const size_t n = (size_t)nbf;
+  constexpr int MA0 = Q0 / C::NSB, MA1 = (Q1 - 1) / C::NSB;
+  constexpr int NMA = MA1 - MA0 + 1;
+  // density blocks needed by the digestion
+  double Pkl[C::NKETS];
+  sfor<C::NSC>([&](auto c) { sfor<C::NSD>([&](auto d) { Pkl[c * C::NSD + d] = P[(k0 + c) * n + l0 + d]; }); });
+  double jkl[C::NKETS];
+  sfor<C::NKETS>([&](auto x) { jkl[x] = 0.0; });
+  double kik[NMA][C::NSC], kil[NMA][C::NSD], kjk[C::NSB][C::NSC], kjl[C::NSB][C::NSD];
+  sfor<NMA>([&](auto a) {
+    sfor<C::NSC>([&](auto c) { kik[a][c] = 0.0; });
+    sfor<C::NSD>([&](auto d) { kil[a][d] = 0.0; });
+  });
+  sfor<C::NSB>([&](auto b) {
+    sfor<C::NSC>([&](auto c) { kjk[b][c] = 0.0; });
+    sfor<C::NSD>([&](auto d) { kjl[b][d] = 0.0; });
+  });
+  const double kq = 0.25 * kscale;
+  sfor<NQ>([&](auto qq) {
+    constexpr int qi = Q0 + qq;
+    constexpr int ma = qi / C::NSB, mb = qi % C::NSB;
+    constexpr int ia = ma - MA0;
+    const size_t ri = (size_t)(i0 + ma) * n, rj = (size_t)(j0 + mb) * n;
+    double v[C::NKETS];
+    ket_to_sph<C>(acc[qq], v);
+    const double Pij = P[ri + j0 + mb];
+    double jij = 0.0;
+    sfor<C::NSC>([&](auto c) {
+      const double Pik = P[ri + k0 + c], Pjk = P[rj + k0 + c];
+      sfor<C::NSD>([&](auto d) {
+        const double x = v[c * C::NSD + d] * deg;
+        const double Pjl = P[rj + l0 + d], Pil = P[ri + l0 + d];
+        jij = fma(Pkl[c * C::NSD + d], x, jij);
+        jkl[c * C::NSD + d] = fma(Pij, x, jkl[c * C::NSD + d]);
+        kik[ia][c] = fma(Pjl, x, kik[ia][c]);
+        kjk[mb][c] = fma(Pil, x, kjk[mb][c]);
+        kil[ia][d] = fma(Pjk, x, kil[ia][d]);
+        kjl[mb][d] = fma(Pik, x, kjl[mb][d]);
+      });
+    });
+    unsafeAtomicAdd(&J[ri + j0 + mb], jij);
+  });
+  sfor<C::NSC>([&](auto c) {
+    sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&J[(k0 + c) * n + l0 + d], jkl[c * C::NSD + d]); });
+  });
+  if (kscale != 0.0) {
+    sfor<NMA>([&](auto a) {
+      const size_t ri = (size_t)(i0 + MA0 + a) * n;
+      sfor<C::NSC>([&](auto c) { unsafeAtomicAdd(&K[ri + k0 + c], kq * kik[a][c]); });
+      sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&K[ri + l0 + d], kq * kil[a][d]); });
+    });
+    // bra-b rows touched by this chunk
+    constexpr int MB_LO = (NMA > 1) ? 0 : (Q0 % C::NSB);
+    constexpr int MB_HI = (NMA > 1) ? C::NSB - 1 : ((Q1 - 1) % C::NSB);
+    sfor<MB_HI - MB_LO + 1>([&](auto bb) {
+      constexpr int b = MB_LO + bb;
+      const size_t rj = (size_t)(j0 + b) * n;
+      sfor<C::NSC>([&](auto c) { unsafeAtomicAdd(&K[rj + k0 + c], kq * kjk[b][c]); });
+      sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&K[rj + l0 + d], kq * kjl[b][d]); });
+    });
+  }
+}
+
+// ------------------------------------------------------------ Schwarz kernel
+// q2[pair] += sum of squares of this chunk of (ab|ab)
+template <class C, int Q0, int Q1>
+__global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ plist, int npair,
+                                                 const PairHdr* __restrict__ pairs,
+                                                 const double4* __restrict__ ppos,
+                                                 const double* __restrict__ pk, RysArgs ra,
+                                                 double* __restrict__ q2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npair) return;
+  const int p = plist[t];
+  const PairHdr pr = pairs[p];
+  double acc[Q1 - Q0][C::NKETC];
+  contract_chunk<C, Q0, Q1>(pr, pr, ppos, pk, ra, acc);
+  double s = 0.0;
+  sfor<Q1 - Q0>([&](auto qq) {
+    double v[C::NKETS];
+    ket_to_sph<C>(acc[qq], v);
+    sfor<C::NKETS>([&](auto x) { s = fma(v[x], v[x], s); });
+  });
+  unsafeAtomicAdd(&q2[p], s);
+}
+
+// ------------------------------------------------------------ launchers
+template <int LA, int LB, int LC, int LD, bool PURE>
+struct ClassOps {
+  using C = Cls<LA, LB, LC, LD, PURE>;
+  static constexpr int CH = chunk_size<C>();
+  static constexpr int NCH = (C::NBRA + CH - 1) / CH;
+
+  static void jk(const JKLaunch& a, hipStream_t s) {
+    if (a.ntask <= 0) return;
+    const int block = 256;
+    const long long grid = (a.ntask + block - 1) / block;
+    hsfor<NCH>([&](auto c) {
+      constexpr int q0 = c * CH;
+      constexpr int q1 = (q0 + CH < C::NBRA) ? q0 + CH : C::NBRA;
+      hipLaunchKernelGGL((k_eri_jk<C, q0, q1>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
+                         a.ntask, a.pairs, a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale);
+    });
+  }
+  static void schwarz(const SchwarzLaunch& a, hipStream_t s) {
+    if (a.npair <= 0) return;
+    const int block = 256;
+    const int grid = (a.npair + block - 1) / block;
+    hsfor<NCH>([&](auto c) {
+      constexpr int q0 = c * CH;
+      constexpr int q1 = (q0 + CH < C::NBRA) ? q0 + CH : C::NBRA;
+      hipLaunchKernelGGL((k_schwarz<C, q0, q1>), dim3(grid), dim3(block), 0, s, a.plist, a.npair,
+                         a.pairs, a.ppos, a.pk, *a.ra, a.q2);
+    });
+  }
+};
+
+}  // namespace qfe

@@ -1,0 +1,285 @@
+// Device-side core of the Fock-build engine: compile-time-specialised Rys
+// quadrature for one shell quartet class (LA LB | LC LD).
+//
+// One thread evaluates one shell quartet (bra pair, ket pair) -- or, for the
+// wide classes, one chunk of bra function pairs of it -- contracting over all
+// primitive quartets in registers:
+//   per primitive quartet: Gaussian product data from the pair tables, Rys
+//   roots/weights (piecewise polynomial table or large-T Hermite limit), the
+//   2-D recursion for G(i,k) along x, y, z, 1-D horizontal transfer to
+//   I(a,b,c,d), and the assembly of the chunk's (ab|cd) in cartesians on the
+//   ket and solid harmonics on the bra.
+// Every index below is a compile-time constant (sfor over integer sequences)
+// so all arrays live in VGPRs and the compiler removes unused terms.
+//
+// Recurrences: Rys, Dupuis & King, J. Comput. Chem. 4, 154 (1983);
+// horizontal transfer: Head-Gordon & Pople, JCP 89, 5777 (1988), applied per
+// cartesian direction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+#include <utility>
+
+#include "sph_coef.inc"
+
+namespace qfe {
+
+#define QFE_HD __host__ __device__
+#define QFE_DI __device__ __forceinline__
+
+// ---------------------------------------------------------------- indices
+QFE_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+QFE_HD constexpr int nsph(int l, bool pure) { return (pure && l >= 2) ? 2 * l + 1 : ncart(l); }
+// CCA order inside a shell: lx descending, then ly descending.
+QFE_HD constexpr int cart_x(int l, int c) {
+  int k = 0;
+  while ((k + 1) * (k + 2) / 2 <= c) ++k;
+  return l - k;
+}
+QFE_HD constexpr int cart_z(int l, int c) {
+  int k = 0;
+  while ((k + 1) * (k + 2) / 2 <= c) ++k;
+  return c - k * (k + 1) / 2;
+}
+QFE_HD constexpr int cart_y(int l, int c) { return l - cart_x(l, c) - cart_z(l, c); }
+
+template <typename F, int... Is>
+QFE_DI void sfor_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+// compile-time loop: f(integral_constant<int, i>) for i = 0..N-1
+template <int N, typename F>
+QFE_DI void sfor(F&& f) {
+  sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// solid-harmonic coefficient (bra side) or identity for cartesian / l < 2
+QFE_HD constexpr double sph_coef(int l, bool pure, int m, int c) {
+  if (!(pure && l >= 2)) return m == c ? 1.0 : 0.0;
+  return kSph[l][m][c];
+}
+
+// ------------------------------------------------------------ device data
+struct PairHdr {        // one significant shell pair (80 B)
+  double A[3];          // centre of shell a
+  double B[3];          // centre of shell b
+  int prim0, nprim;     // range in the primitive-pair arrays
+  int sa, sb;           // shell indices (AO order of the reference)
+  int bfa, bfb;         // first AO of each shell
+  int cls, pad;
+};
+
+// primitive pair: ppos = (p, Px, Py, Pz); pk = c_a c_b exp(-ab/p |AB|^2) / p
+struct RysArgs {
+  const double* coef[9];   // per root count n: [nint][2n][RYS_DEG+1]
+  double tmax[9];
+  double hx2[9][8];        // large-T limit: u_i = hx2/T
+  double hw[9][8];         //               w_i = hw/sqrt(T)
+};
+constexpr int RYS_DEG = 11;
+
+template <int N>
+QFE_DI void rys_roots(double T, const RysArgs& ra, double (&u)[N], double (&w)[N]) {
+  if (T >= ra.tmax[N]) {
+    const double it = 1.0 / T;
+    const double is = sqrt(it);
+    sfor<N>([&](auto i) {
+      u[i] = ra.hx2[N][i] * it;
+      w[i] = ra.hw[N][i] * is;
+    });
+  } else {
+    const int k = (int)T;
+    const double x = T - ((double)k + 0.5);
+    const double* __restrict__ c = ra.coef[N] + (size_t)k * (2 * N) * (RYS_DEG + 1);
+    sfor<N>([&](auto i) {
+      const double* cu = c + i * (RYS_DEG + 1);
+      const double* cw = c + (N + i) * (RYS_DEG + 1);
+      double vu = cu[RYS_DEG], vw = cw[RYS_DEG];
+#pragma unroll
+      for (int d = RYS_DEG - 1; d >= 0; --d) {
+        vu = fma(vu, x, cu[d]);
+        vw = fma(vw, x, cw[d]);
+      }
+      u[i] = vu;
+      w[i] = vw;
+    });
+  }
+}
+
+// ------------------------------------------------------------ class traits
+template <int LA_, int LB_, int LC_, int LD_, bool PURE_>
+struct Cls {
+  static constexpr int LA = LA_, LB = LB_, LC = LC_, LD = LD_;
+  static constexpr bool PURE = PURE_;
+  static constexpr int EA = LA + LB, FC = LC + LD;
+  static constexpr int L = EA + FC;
+  static constexpr int NR = L / 2 + 1;
+  static constexpr int NCA = ncart(LA), NCB = ncart(LB), NCC = ncart(LC), NCD = ncart(LD);
+  static constexpr int NSA = nsph(LA, PURE), NSB = nsph(LB, PURE), NSC = nsph(LC, PURE),
+                       NSD = nsph(LD, PURE);
+  static constexpr int NBRA = NSA * NSB;   // bra function pairs
+  static constexpr int NKETC = NCC * NCD;  // ket cartesian pairs
+  static constexpr int NKETS = NSC * NSD;  // ket function pairs
+};
+
+// 1-D integrals of one direction for one root: I[a][b][c][d]
+template <class C>
+struct OneD {
+  double v[C::LA + 1][C::LB + 1][C::LC + 1][C::LD + 1];
+};
+
+// 2-D recursion + 1-D horizontal transfer for one direction
+template <class C>
+QFE_DI void rys_1d(double s, double C00, double C00p, double B00, double B10, double B01,
+                   double AB, double CD, OneD<C>& out) {
+  constexpr int EA = C::EA, FC = C::FC;
+  double g[EA + 1][FC + 1];
+  g[0][0] = s;
+  if constexpr (EA >= 1) g[1][0] = C00 * s;
+  sfor<(EA >= 2 ? EA - 1 : 0)>([&](auto ii) {
+    constexpr int i = ii + 1;
+    g[i + 1][0] = C00 * g[i][0] + (double)i * B10 * g[i - 1][0];
+  });
+  if constexpr (FC >= 1) {
+    g[0][1] = C00p * s;
+    sfor<EA>([&](auto ii) {
+      constexpr int i = ii + 1;
+      g[i][1] = C00p * g[i][0] + (double)i * B00 * g[i - 1][0];
+    });
+    sfor<(FC >= 2 ? FC - 1 : 0)>([&](auto kk) {
+      constexpr int k = kk + 1;
+      g[0][k + 1] = C00p * g[0][k] + (double)k * B01 * g[0][k - 1];
+      sfor<EA>([&](auto ii) {
+        constexpr int i = ii + 1;
+        g[i][k + 1] = C00p * g[i][k] + (double)k * B01 * g[i][k - 1] + (double)i * B00 * g[i - 1][k];
+      });
+    });
+  }
+  // ket transfer: h[i][c][d] for i <= EA, from g[i][c + d]
+  double h[EA + 1][C::LC + 1][C::LD + 1];
+  sfor<EA + 1>([&](auto i) {
+    // level d = 0
+    sfor<C::LC + 1>([&](auto c) { h[i][c][0] = g[i][c]; });
+    // d > 0: (c, d) = (c+1, d-1) + CD (c, d-1); needs c + d <= FC
+    double t[FC + 1][C::LD + 1];
+    sfor<FC + 1>([&](auto k) { t[k][0] = g[i][k]; });
+    sfor<C::LD>([&](auto dd) {
+      constexpr int d = dd + 1;
+      sfor<FC + 1 - d>([&](auto k) { t[k][d] = t[k + 1][d - 1] + CD * t[k][d - 1]; });
+    });
+    sfor<C::LC + 1>([&](auto c) { sfor<C::LD + 1>([&](auto d) { h[i][c][d] = t[c][d]; }); });
+  });
+  // bra transfer: I[a][b][c][d] from h[a + b][c][d]
+  sfor<C::LC + 1>([&](auto c) {
+    sfor<C::LD + 1>([&](auto d) {
+      double t[EA + 1][C::LB + 1];
+      sfor<EA + 1>([&](auto i) { t[i][0] = h[i][c][d]; });
+      sfor<C::LB>([&](auto bb) {
+        constexpr int b = bb + 1;
+        sfor<EA + 1 - b>([&](auto i) { t[i][b] = t[i + 1][b - 1] + AB * t[i][b - 1]; });
+      });
+      sfor<C::LA + 1>([&](auto a) { sfor<C::LB + 1>([&](auto b) { out.v[a][b][c][d] = t[a][b]; }); });
+    });
+  });
+}
+
+// Contract all primitive quartets of (bra|ket) into acc[q][kc] for the bra
+// function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and all ket
+// cartesian pairs kc (row-major over (c, d)).
+template <class C, int Q0, int Q1>
+QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket, const double4* __restrict__ ppos,
+                           const double* __restrict__ pk, const RysArgs& ra,
+                           double (&acc)[Q1 - Q0][C::NKETC]) {
+  constexpr double TWO_PI_52 = 34.98683665524972;  // 2 pi^(5/2)
+  sfor<Q1 - Q0>([&](auto q) { sfor<C::NKETC>([&](auto k) { acc[q][k] = 0.0; }); });
+  const double ABx = bra.A[0] - bra.B[0], ABy = bra.A[1] - bra.B[1], ABz = bra.A[2] - bra.B[2];
+  const double CDx = ket.A[0] - ket.B[0], CDy = ket.A[1] - ket.B[1], CDz = ket.A[2] - ket.B[2];
+  for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
+    const double4 P = ppos[ib];
+    const double Kb = pk[ib];
+    const double p = P.x;
+    const double PAx = P.y - bra.A[0], PAy = P.z - bra.A[1], PAz = P.w - bra.A[2];
+    for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
+      const double4 Q = ppos[ik];
+      const double Kk = pk[ik];
+      const double q = Q.x;
+      const double QCx = Q.y - ket.A[0], QCy = Q.z - ket.A[1], QCz = Q.w - ket.A[2];
+      const double PQx = P.y - Q.y, PQy = P.z - Q.z, PQz = P.w - Q.w;
+      const double rpq = 1.0 / (p + q);
+      const double rho = p * q * rpq;
+      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = TWO_PI_52 * Kb * Kk * sqrt(rpq);
+      double u[C::NR], w[C::NR];
+      rys_roots<C::NR>(T, ra, u, w);
+      const double qr = q * rpq, pr = p * rpq;
+      const double h2p = 0.5 / p, h2q = 0.5 / q;
+#pragma unroll 1
+      for (int r = 0; r < C::NR; ++r) {
+        const double ur = u[r];
+        const double B00 = 0.5 * rpq * ur;
+        const double B10 = h2p - h2p * qr * ur;
+        const double B01 = h2q - h2q * pr * ur;
+        const double xa = qr * ur, xc = pr * ur;
+        OneD<C> Ix, Iy, Iz;
+        rys_1d<C>(1.0, PAx - xa * PQx, QCx + xc * PQx, B00, B10, B01, ABx, CDx, Ix);
+        rys_1d<C>(1.0, PAy - xa * PQy, QCy + xc * PQy, B00, B10, B01, ABy, CDy, Iy);
+        rys_1d<C>(w[r] * pref, PAz - xa * PQz, QCz + xc * PQz, B00, B10, B01, ABz, CDz, Iz);
+        // assemble: bra in output functions, ket in cartesians
+        sfor<Q1 - Q0>([&](auto qq) {
+          constexpr int qi = Q0 + qq;
+          constexpr int ma = qi / C::NSB, mb = qi % C::NSB;
+          sfor<C::NCC>([&](auto cc) {
+            constexpr int cx = cart_x(C::LC, cc), cy = cart_y(C::LC, cc), cz = cart_z(C::LC, cc);
+            sfor<C::NCD>([&](auto dc) {
+              constexpr int dx = cart_x(C::LD, dc), dy = cart_y(C::LD, dc), dz = cart_z(C::LD, dc);
+              double s = 0.0;
+              sfor<C::NCA>([&](auto ac) {
+                constexpr double ca = sph_coef(C::LA, C::PURE, ma, ac);
+                if constexpr (ca != 0.0) {
+                  constexpr int ax = cart_x(C::LA, ac), ay = cart_y(C::LA, ac), az = cart_z(C::LA, ac);
+                  sfor<C::NCB>([&](auto bc) {
+                    constexpr double cb = sph_coef(C::LB, C::PURE, mb, bc);
+                    if constexpr (cb != 0.0) {
+                      constexpr int bx = cart_x(C::LB, bc), by = cart_y(C::LB, bc), bz = cart_z(C::LB, bc);
+                      const double v = Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz];
+                      if constexpr (ca * cb == 1.0)
+                        s += v;
+                      else
+                        s = fma(ca * cb, v, s);
+                    }
+                  });
+                }
+              });
+              acc[qq][cc * C::NCD + dc] += s;
+            });
+          });
+        });
+      }
+    }
+  }
+}
+
+// ket cartesian -> output functions for one bra function pair
+template <class C>
+QFE_DI void ket_to_sph(const double (&in)[C::NKETC], double (&out)[C::NKETS]) {
+  sfor<C::NSC>([&](auto mc) {
+    sfor<C::NSD>([&](auto md) {
+      double s = 0.0;
+      sfor<C::NCC>([&](auto cc) {
+        constexpr double c1 = sph_coef(C::LC, C::PURE, mc, cc);
+        if constexpr (c1 != 0.0) {
+          sfor<C::NCD>([&](auto dc) {
+            constexpr double c2 = sph_coef(C::LD, C::PURE, md, dc);
+            if constexpr (c2 != 0.0) s = fma(c1 * c2, in[cc * C::NCD + dc], s);
+          });
+        }
+      });
+      out[mc * C::NSD + md] = s;
+    });
+  });
+}
+
+}  // namespace qfe

@@ -1,0 +1,189 @@
+// HIP kernels of the Fock-build engine (gfx950).
+//
+//   k_shell_norm     shell-block infinity norms of the density
+//                    (compute_shellblock_norm, libint2_direct.cpp:43-59)
+//   k_task_count /   two-pass construction of the screened canonical quartet
+//   k_task_write     list, grouped by class (the loop + test of
+//                    libint2_direct.cpp:357-411)
+//   k_symmetrize     J = (J+J^T)/4, K = (K+K^T)/2 (libint2_direct.cpp:594-625)
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "eri_core.hpp"
+#include "kernels.hpp"
+
+namespace qfe {
+
+// ------------------------------------------------------------ density norms
+__global__ void k_shell_norm(const double* __restrict__ P, int nbf, const int* __restrict__ sh_bf0,
+                             const int* __restrict__ sh_nf, int nsh, double* __restrict__ Pn,
+                             int* __restrict__ bad) {
+  const int s2 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s1 = blockIdx.y;
+  if (s2 >= nsh) return;
+  const int r0 = sh_bf0[s1], c0 = sh_bf0[s2], nr = sh_nf[s1], nc = sh_nf[s2];
+  double m = 0.0;
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < nc; ++j) {
+      const double v = fabs(P[(size_t)(r0 + i) * nbf + c0 + j]);
+      if (!(v <= 1.0e300)) *bad = 1;  // NaN or Inf
+      m = fmax(m, v);
+    }
+  Pn[(size_t)s1 * nsh + s2] = m;
+}
+
+__global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __restrict__ out) {
+  __shared__ double sm[256];
+  double m = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, x[i]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // values are non-negative: compare the bit patterns as unsigned integers
+    atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(sm[0]));
+  }
+}
+
+// ------------------------------------------------------------ task list
+// work item = (bra pair p, ket bucket b); one wavefront scans bucket b.
+__device__ __forceinline__ bool quartet_passes(const PairHdr& P1, double Q1, const PairHdr& P2,
+                                               double Q2, const double* __restrict__ Pn, int nsh,
+                                               double thr) {
+  const size_t a = P1.sa, b = P1.sb, c = P2.sa, d = P2.sb;
+  double m = Pn[a * nsh + b];
+  m = fmax(m, Pn[c * nsh + d]);
+  m = fmax(m, Pn[a * nsh + c]);
+  m = fmax(m, Pn[a * nsh + d]);
+  m = fmax(m, Pn[b * nsh + c]);
+  m = fmax(m, Pn[b * nsh + d]);
+  return !(m * Q1 * Q2 < thr);
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ items, long long nitems,
+                                                   const int2* __restrict__ buckets,
+                                                   const int* __restrict__ pair_bucket,
+                                                   const PairHdr* __restrict__ pairs,
+                                                   const double* __restrict__ Q,
+                                                   const double* __restrict__ Pn, int nsh,
+                                                   const double* __restrict__ pmax_ptr, double thr,
+                                                   int* __restrict__ counts,
+                                                   const long long* __restrict__ offsets,
+                                                   int2* __restrict__ tasks) {
+  const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (item >= nitems) return;
+  const int2 it = items[item];
+  const int p = it.x;
+  const int2 bk = buckets[it.y];
+  int end = bk.y;
+  if (pair_bucket[p] == it.y) end = p + 1;  // same bucket: q <= p
+  const double Qp = Q[p];
+  const PairHdr Pp = pairs[p];
+  const double cut = thr / (*pmax_ptr);
+  long long base = WRITE ? offsets[item] : 0;
+  int count = 0;
+  for (int q0 = bk.x; q0 < end; q0 += 64) {
+    // sorted by Q descending: if the first ket fails the Schwarz bound, all later do
+    if (Qp * Q[q0] < cut) break;
+    const int q = q0 + lane;
+    bool ok = false;
+    if (q < end) {
+      const double Qq = Q[q];
+      ok = quartet_passes(Pp, Qp, pairs[q], Qq, Pn, nsh, thr);
+    }
+    const unsigned long long m = __ballot(ok);
+    if (WRITE && ok) {
+      const int pos = __popcll(m & ((1ull << lane) - 1ull));
+      tasks[base + count + pos] = make_int2(p, q);
+    }
+    count += __popcll(m);
+  }
+  if (!WRITE && lane == 0) counts[item] = count;
+}
+
+__global__ void k_gather_offsets(const long long* __restrict__ offsets, const long long* __restrict__ idx,
+                                 int n, long long* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = offsets[idx[i]];
+}
+
+// ------------------------------------------------------------ symmetrisation
+__global__ void k_symmetrize(double* __restrict__ J, double* __restrict__ K, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j > i || i >= n) return;
+  const size_t a = (size_t)i * n + j, b = (size_t)j * n + i;
+  if (J) {
+    const double v = 0.25 * (J[a] + J[b]);
+    J[a] = v;
+    J[b] = v;
+  }
+  if (K) {
+    const double v = 0.5 * (K[a] + K[b]);
+    K[a] = v;
+    K[b] = v;
+  }
+}
+
+// ------------------------------------------------------------ host wrappers
+void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
+                       double* Pn, int* bad, double* pmax, hipStream_t s) {
+  dim3 block(64), grid((nsh + 63) / 64, nsh);
+  hipLaunchKernelGGL(k_shell_norm, grid, block, 0, s, P, nbf, sh_bf0, sh_nf, nsh, Pn, bad);
+  (void)hipMemsetAsync(pmax, 0, sizeof(double), s);
+  size_t n = (size_t)nsh * nsh;
+  int g = (int)((n + 255) / 256);
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_max_reduce, dim3(g), dim3(256), 0, s, Pn, n, pmax);
+}
+
+void launch_task_count(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
+                       const PairHdr* pairs, const double* Q, const double* Pn, int nsh,
+                       const double* pmax, double thr, int* counts, hipStream_t s) {
+  if (nitems <= 0) return;
+  const long long threads = nitems * 64;
+  const long long grid = (threads + 255) / 256;
+  hipLaunchKernelGGL((k_task_scan<false>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
+                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, counts, (const long long*)nullptr,
+                     (int2*)nullptr);
+}
+
+void launch_task_write(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
+                       const PairHdr* pairs, const double* Q, const double* Pn, int nsh,
+                       const double* pmax, double thr, const long long* offsets, int2* tasks,
+                       hipStream_t s) {
+  if (nitems <= 0) return;
+  const long long threads = nitems * 64;
+  const long long grid = (threads + 255) / 256;
+  hipLaunchKernelGGL((k_task_scan<true>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
+                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, (int*)nullptr, offsets, tasks);
+}
+
+size_t scan_temp_bytes(long long n) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int*)nullptr, (long long*)nullptr, (int)n);
+  return bytes;
+}
+
+void launch_exclusive_scan(void* temp, size_t temp_bytes, const int* in, long long* out, long long n,
+                           hipStream_t s) {
+  (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int)n, s);
+}
+
+void launch_gather_offsets(const long long* offsets, const long long* idx, int n, long long* out,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_offsets, dim3((n + 63) / 64), dim3(64), 0, s, offsets, idx, n, out);
+}
+
+void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
+  dim3 block(256), grid((n + 255) / 256, n);
+  hipLaunchKernelGGL(k_symmetrize, grid, block, 0, s, J, K, n);
+}
+
+}  // namespace qfe

@@ -1,0 +1,126 @@
+"""ERI engine interface of the SCF library, MI355X implementation.
+
+Mirrors qdk::chemistry::scf::ERI (scf/include/qdk/chemistry/scf/core/eri.h:
+23-176): `build_JK(P, alpha, beta, omega)` (:79-80) and
+`get_gradients(P, alpha, beta, omega)` (:100-101), created by `create`
+switching on `ERIConfig.method` like ERI::create (scf/src/eri/eri.cpp:36-68).
+The engine fills the `ERIMethod.Rys` slot (core/enums.h:22-35); the
+reference's other methods (Libint2Direct, Incore, SnK) are CPU/third-party
+engines outside this package, and requesting them raises, as an unknown
+method does in the reference (eri.cpp:65).
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+class ERIMethod(enum.IntEnum):
+    """Same values as core/enums.h:22-35."""
+    Libint2Direct = 0
+    Rys = 1
+    HGP = 2
+    Incore = 3
+    SnK = 4
+
+
+@dataclass
+class ERIConfig:
+    """Field names of ERIConfig (core/scf.h:65-94) plus the engine knobs."""
+    method: ERIMethod = ERIMethod.Rys
+    eri_threshold: float = 1e-12
+    shell_pair_threshold: float = 1e-12
+    use_atomics: bool = True
+    is_mix_precision: bool = False
+    mix_precision_threshold: float = 1e-6
+    device_id: int = 0
+    verbose: int = 0
+
+
+class RysERI:
+    """GPU Fock-build engine (libqfe.so) for one basis/structure."""
+
+    def __init__(self, basis, structure, cfg: ERIConfig | None = None):
+        cfg = cfg or ERIConfig()
+        self.cfg = cfg
+        self.basis = basis
+        self.structure = structure
+        self.nbf = basis.nbf
+        self._b = _lib.BasisArrays(basis, structure)
+        c = _lib.QfeConfig(cfg.eri_threshold, cfg.shell_pair_threshold, cfg.device_id,
+                           int(cfg.is_mix_precision), cfg.mix_precision_threshold, cfg.verbose)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().qfe_create(ctypes.byref(self._b.c), ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            _lib.lib().qfe_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0):
+        """(J, K) for a host density (ndm x nbf x nbf or nbf x nbf)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        ndm = P.size // (self.nbf * self.nbf)
+        J = np.empty_like(P)
+        K = np.empty_like(P)
+        _lib.check(_lib.lib().qfe_build_jk(self.h, _lib.ptr(P), _lib.ptr(J), _lib.ptr(K), alpha, beta, omega, ndm))
+        return J, K
+
+    def build_JK_device(self, dP, dJ, dK, alpha=1.0, beta=0.0, omega=0.0, stream=None):
+        """Device-resident variant: dP, dJ, dK are torch CUDA (HIP) tensors."""
+        s = ctypes.c_void_p(stream) if stream is not None else None
+        _lib.check(_lib.lib().qfe_build_jk_device(self.h, ctypes.c_void_p(dP.data_ptr()),
+                                                  ctypes.c_void_p(dJ.data_ptr()) if dJ is not None else None,
+                                                  ctypes.c_void_p(dK.data_ptr()) if dK is not None else None,
+                                                  alpha, beta, omega, s))
+
+    def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0):
+        """(dJ, dK): 3*natoms each, xyz-major (libint2_direct.cpp:824)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        na = self.structure.natoms
+        dJ = np.zeros(3 * na)
+        dK = np.zeros(3 * na)
+        _lib.check(_lib.lib().qfe_gradients(self.h, _lib.ptr(P), _lib.ptr(dJ), _lib.ptr(dK), alpha, beta, omega))
+        return dJ, dK
+
+    def stats(self) -> dict:
+        s = _lib.QfeStats()
+        _lib.check(_lib.lib().qfe_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.QfeStats._fields_}
+
+
+def create(basis, structure, cfg: ERIConfig | None = None):
+    """ERI::create (eri.cpp:36-68): dispatch on cfg.method."""
+    cfg = cfg or ERIConfig()
+    if cfg.method == ERIMethod.Rys:
+        return RysERI(basis, structure, cfg)
+    raise ValueError(f"Invalid ERI method {cfg.method!r} for the MI355X engine")
+
+
+def one_body(basis, structure, op: int) -> np.ndarray:
+    """S (0), T (1) or V (2) on the host."""
+    b = _lib.BasisArrays(basis, structure)
+    out = np.zeros((basis.nbf, basis.nbf))
+    _lib.check(_lib.lib().qfe_one_body(ctypes.byref(b.c), op, _lib.ptr(out)))
+    return out
+
+
+def one_body_grad(basis, structure, op: int, W) -> np.ndarray:
+    """sum_mn W_mn dO_mn/dR, natoms x 3."""
+    b = _lib.BasisArrays(basis, structure)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    g = np.zeros((structure.natoms, 3))
+    _lib.check(_lib.lib().qfe_one_body_grad(ctypes.byref(b.c), op, _lib.ptr(W), _lib.ptr(g)))
+    return g

@@ -98,7 +98,9 @@ def water_cluster(nmol: int, seed: int, spacing: float = 3.0, jitter: float = 0.
 
     Lattice sites fill x fastest: 16 -> 4x2x2, 64 -> 4x4x4.  Each oxygen gets
     U(-jitter, jitter) per axis; each molecule a uniform random rotation about
-    its oxygen, redrawn while any intermolecular H..H < 1.5 A or O..O < 2.5 A.
+    its oxygen.  A molecule's jitter and rotation are redrawn together while
+    any intermolecular H..H < 1.5 A or O..O < 2.5 A (an O..O clash cannot be
+    cured by rotation alone: 3.0 - 2*0.3 < 2.5).
     """
     dims = {1: (1, 1, 1), 2: (2, 1, 1), 4: (2, 2, 1), 8: (2, 2, 2), 16: (4, 2, 2), 32: (4, 4, 2), 64: (4, 4, 4)}[nmol]
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -108,8 +110,9 @@ def water_cluster(nmol: int, seed: int, spacing: float = 3.0, jitter: float = 0.
         ix = k % dims[0]
         iy = (k // dims[0]) % dims[1]
         iz = k // (dims[0] * dims[1])
-        o = np.array([ix, iy, iz], dtype=np.float64) * spacing + rng.uniform(-jitter, jitter, 3)
+        site = np.array([ix, iy, iz], dtype=np.float64) * spacing
         for _ in range(1000):
+            o = site + rng.uniform(-jitter, jitter, 3)
             R = _quat_rotation(*rng.uniform(0.0, 1.0, 3))
             mol = o + mono @ R.T
             ok = True

@@ -1,0 +1,153 @@
+"""RHF driver around the GPU Fock build.
+
+Restates the Fock-build step of SCFImpl::iterate_
+(scf/src/scf/scf_impl.cpp:495-641):
+  * P_diff = P for step < incremental_fock_start_step (3) or every
+    fock_reset_steps, with F reset to H; otherwise P_diff = P - P_last (:568-578)
+  * J, K = build_JK(P_diff, 1, 0, 0) (:581); F += J - 1/2 K (:363-391)
+  * E = E_nuc + Tr(P H) + 1/2 Tr(P (F - H)) (:402-420)
+  * converged when ||P - P_last||_F / N < density_threshold and
+    ||FPS - SPF||_inf / N < og_threshold (scf_algorithm.cpp:440-495), with
+    density_threshold = 1e2 * convergence_threshold and og_threshold =
+    convergence_threshold (ms/scf.cpp:167-174)
+  * orthogonalisation X = U s^-1/2 dropping s < max(s)/1e9 (:422-493)
+The orbital update is DIIS (scf_algorithm/diis.cpp) in FP64 on the host; the
+reference's default switches to GDM near convergence, which changes the
+iteration path but not the converged energy.  After convergence a full
+(non-incremental) rebuild gives the final Fock matrix (:612-621).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .eri import ERIConfig, create, one_body
+
+
+@dataclass
+class SCFConfig:
+    convergence_threshold: float = 1e-7
+    max_iterations: int = 100
+    incremental_fock_start_step: int = 3
+    fock_reset_steps: int = 1073741824
+    diis_size: int = 10
+    lindep_threshold: float = -1.0
+    fp64_final_iterations: int = 3      # mixed precision: last builds in FP64
+    eri: ERIConfig = field(default_factory=ERIConfig)
+
+
+def orthogonalizer(S, lindep=-1.0):
+    s, U = np.linalg.eigh(S)
+    thr = lindep if lindep >= 0 else s.max() / 1e9
+    keep = s >= thr
+    return U[:, keep] / np.sqrt(s[keep])
+
+
+class RHF:
+    def __init__(self, structure, basis, cfg: SCFConfig | None = None, eri=None):
+        self.structure = structure
+        self.basis = basis
+        self.cfg = cfg or SCFConfig()
+        self.eri = eri or create(basis, structure, self.cfg.eri)
+        self.nocc = structure.nelectrons // 2
+        if structure.nelectrons % 2:
+            raise ValueError("RHF needs an even number of electrons")
+        self.S = one_body(basis, structure, 0)
+        self.H = one_body(basis, structure, 1) + one_body(basis, structure, 2)
+        self.X = orthogonalizer(self.S, self.cfg.lindep_threshold)
+        self.E_nuc = structure.nuclear_repulsion()
+        self.build_times = []
+        self.build_stats = []
+
+    def _diag(self, F):
+        e, Cp = np.linalg.eigh(self.X.T @ F @ self.X)
+        C = self.X @ Cp
+        return e, C
+
+    def _density(self, C):
+        Co = C[:, : self.nocc]
+        return 2.0 * Co @ Co.T
+
+    def _jk(self, P):
+        t = time.perf_counter()
+        J, K = self.eri.build_JK(P, 1.0, 0.0, 0.0)
+        self.build_times.append(time.perf_counter() - t)
+        if hasattr(self.eri, "stats"):
+            self.build_stats.append(self.eri.stats())
+        return J, K
+
+    def energy(self, P, F):
+        return self.E_nuc + float(np.sum(P * self.H)) + 0.5 * float(np.sum(P * (F - self.H)))
+
+    def run(self, P0=None, verbose=False):
+        cfg = self.cfg
+        n = self.basis.nbf
+        if P0 is None:
+            e, C = self._diag(self.H)
+            P = self._density(C)
+        else:
+            P = np.array(P0, dtype=np.float64)
+        conv = cfg.convergence_threshold
+        dens_thr, og_thr = conv * 1e2, conv
+        F = self.H.copy()
+        P_last = np.zeros_like(P)
+        P_prev_conv = np.zeros_like(P)
+        E_last = 0.0
+        diis_F, diis_E = [], []
+        converged = False
+        for step in range(cfg.max_iterations):
+            if step < cfg.incremental_fock_start_step or step % cfg.fock_reset_steps == 0:
+                P_diff = P
+                F = self.H.copy()
+            else:
+                P_diff = P - P_last
+            P_last = P.copy()
+            J, K = self._jk(P_diff)
+            F = F + J - 0.5 * K
+            E = self.energy(P, F)
+            if not np.isfinite(E):
+                raise RuntimeError("NaN or INF Encountered in SCF Energy")
+            FPS = F @ P @ self.S
+            err = FPS - FPS.T
+            og = float(np.max(np.abs(err))) / n
+            drms = float(np.linalg.norm(P_prev_conv - P)) / n
+            P_prev_conv = P.copy()
+            if verbose:
+                print(f"Step {step:03d}: E={E:.15e}, DE={E - E_last:+.3e}, |DP|={drms:.3e}, |DG|={og:.3e}")
+            E_last = E
+            if drms < dens_thr and og < og_thr:
+                converged = True
+                break
+            diis_F.append(F.copy())
+            diis_E.append(err)
+            if len(diis_F) > cfg.diis_size:
+                diis_F.pop(0)
+                diis_E.pop(0)
+            m = len(diis_F)
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(i + 1):
+                    B[i, j] = B[j, i] = float(np.sum(diis_E[i] * diis_E[j]))
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            try:
+                c = np.linalg.solve(B, rhs)[:m]
+            except np.linalg.LinAlgError:
+                c = np.zeros(m)
+                c[-1] = 1.0
+            Fd = sum(ci * Fi for ci, Fi in zip(c, diis_F))
+            e, C = self._diag(Fd)
+            P = self._density(C)
+        if not converged:
+            raise RuntimeError(f"SCF failed to converge after {cfg.max_iterations} steps")
+        # final full rebuild (scf_impl.cpp:612-621)
+        J, K = self._jk(P)
+        F = self.H + J - 0.5 * K
+        E = self.energy(P, F)
+        e, C = self._diag(F)
+        self.result = dict(energy=E, P=P, F=F, C=C, eps=e, iterations=step + 1, converged=converged,
+                           build_seconds=float(np.sum(self.build_times)))
+        return self.result
