@@ -107,6 +107,12 @@ int qfe_one_body(const qfe_basis* basis, int op, double* out);
 int qfe_one_body_grad(const qfe_basis* basis, int op, const double* W, double* grad);
 
 int qfe_get_stats(const qfe_handle* h, qfe_stats* out);
+
+/* Per-class census of the last build: for class pair i < min(cap, n):
+ * cls[4i..4i+3] = (la, lb, lc, ld) as evaluated, quartets[i] and
+ * prim_quartets[i] screened counts.  Returns n (number of class pairs), or
+ * -1 for a null handle.  No reference counterpart: instrumentation. */
+int qfe_class_stats(const qfe_handle* h, int cap, int* cls, double* quartets, double* prim_quartets);
 int qfe_nbf(const qfe_handle* h);
 void qfe_destroy(qfe_handle* h);
 const char* qfe_last_error(void);

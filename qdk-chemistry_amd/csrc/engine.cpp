@@ -191,12 +191,16 @@ struct Engine {
   long long prim_pairs = 0;
 
   DevBuf<PairHdr> pairs_d;
+  DevBuf<int2> psh_d;  // (sa, sb) per sorted pair, for the screening scan
   DevBuf<double4> ppos_d;
   DevBuf<double> pk_d;
   DevBuf<double> Q_d;
   DevBuf<int2> buckets_d;
   DevBuf<int> pair_bucket_d;
   DevBuf<int2> items_d;
+  DevBuf<int> item_cp_d;
+  DevBuf<double> cstat_d;           // [2][ncp]: quartets, primitive quartets
+  std::vector<double> cstat_h;
   DevBuf<int> counts_d;
   DevBuf<long long> offsets_d;
   DevBuf<long long> bidx_d, bval_d;
@@ -208,6 +212,7 @@ struct Engine {
   DevBuf<int> bad_d;
   DevBuf<double> P_d, J_d, K_d;
   std::vector<DevBuf<double>> rys_d;
+  DevBuf<double> boys_d;
   RysArgs ra{};
 
   qfe_stats stats{};
@@ -218,8 +223,33 @@ struct Engine {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // F_m(k*BOYS_DT), m < BOYS_W: series for the top order in long double,
+  // F_m(T) = e^-T sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)), then downward
+  static std::vector<double> boys_table() {
+    std::vector<double> t((size_t)BOYS_NG * BOYS_W);
+    for (int g = 0; g < BOYS_NG; ++g) {
+      const long double T = g * (long double)BOYS_DT;
+      const int mt = BOYS_W - 1;
+      long double term = 1.0L / (2 * mt + 1), sum = term;
+      for (int k = 1; k < 4000; ++k) {
+        term *= 2.0L * T / (2 * mt + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-22L) break;
+      }
+      const long double eT = expl(-T);
+      long double F = eT * sum;
+      t[(size_t)g * BOYS_W + mt] = (double)F;
+      for (int m = mt - 1; m >= 0; --m) {
+        F = (2.0L * T * F + eT) / (2 * m + 1);
+        t[(size_t)g * BOYS_W + m] = (double)F;
+      }
+    }
+    return t;
+  }
+
   void setup_rys() {
     const RysTableHost& t = rys_table();
+    boys_d.upload(boys_table(), stream);
     rys_d.resize(t.nmax + 1);
     std::memset(&ra, 0, sizeof(ra));
     for (int n = 1; n <= t.nmax && n <= 8; ++n) {
@@ -231,6 +261,7 @@ struct Engine {
         ra.hw[n][i] = t.hw[n][i];
       }
     }
+    ra.boys = boys_d.p;
   }
 
   void build_pairs() {
@@ -376,12 +407,21 @@ struct Engine {
     cp_item_begin[ncp] = (int)items_h.size();
 
     pairs_d.upload(pairs_h, stream);
+    std::vector<int2> psh(npair);
+    for (int i = 0; i < npair; ++i) psh[i] = make_int2(pairs_h[i].sa, pairs_h[i].sb);
+    psh_d.upload(psh, stream);
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
     Q_d.upload(Q_h, stream);
     buckets_d.upload(buckets_h, stream);
     pair_bucket_d.upload(pair_bucket_h, stream);
     items_d.upload(items_h, stream);
+    std::vector<int> item_cp(items_h.size());
+    for (int cp = 0; cp < ncp; ++cp)
+      for (int i = cp_item_begin[cp]; i < cp_item_begin[cp + 1]; ++i) item_cp[i] = cp;
+    item_cp_d.upload(item_cp, stream);
+    cstat_d.alloc(2 * (size_t)ncp);
+    cstat_h.assign(2 * (size_t)ncp, 0.0);
     const long long nitems = (long long)items_h.size();
     counts_d.alloc(nitems + 1);
     HIP_CHECK(hipMemsetAsync(counts_d.p, 0, sizeof(int) * (nitems + 1), stream));
@@ -446,23 +486,28 @@ struct Engine {
     launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, bad_d.p, pmax_d.p, s);
     HIP_CHECK(hipGetLastError());
     const long long nitems = (long long)items_h.size();
-    launch_task_count(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, pairs_d.p, Q_d.p, Pn_d.p, nsh,
+    launch_task_count(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
                       pmax_d.p, eri_thr, counts_d.p, s);
     HIP_CHECK(hipGetLastError());
     launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, nitems + 1, s);
     HIP_CHECK(hipGetLastError());
     const int ncp = (int)cp_bra.size();
+    HIP_CHECK(hipMemsetAsync(cstat_d.p, 0, sizeof(double) * 2 * ncp, s));
+    launch_class_stats(counts_d.p, items_d.p, nitems, item_cp_d.p, buckets_d.p, pairs_d.p, ncp, cstat_d.p,
+                       cstat_d.p + ncp, s);
+    HIP_CHECK(hipGetLastError());
     launch_gather_offsets(offsets_d.p, bidx_d.p, ncp + 1, bval_d.p, s);
     std::vector<long long> bval(ncp + 1);
     int bad = 0;
     HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * (ncp + 1), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&bad, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(cstat_h.data(), cstat_d.p, sizeof(double) * 2 * ncp, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (bad) throw std::runtime_error("Density matrix contains NaN/Inf values.");
     const long long total = bval[ncp];
     if (total > 0) tasks_d.alloc((size_t)total);
-    launch_task_write(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, pairs_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, offsets_d.p, tasks_d.p, s);
+    launch_task_write(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
+                      pmax_d.p, eri_thr, counts_d.p, offsets_d.p, tasks_d.p, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ev[1], s));
     if (dJ) HIP_CHECK(hipMemsetAsync(dJ, 0, nn * sizeof(double), s));
@@ -500,6 +545,9 @@ struct Engine {
     HIP_CHECK(hipEventElapsedTime(&t03, ev[0], ev[3]));
     stats.quartets = nq;
     stats.quartets_fp32 = 0;
+    double npq = 0.0;
+    for (int cp = 0; cp < ncp; ++cp) npq += cstat_h[ncp + cp];
+    stats.prim_quartets = (long long)npq;
     stats.ms_screen = t01;
     stats.ms_eri = t12;
     stats.ms_total = t03;
@@ -606,6 +654,27 @@ int qfe_get_stats(const qfe_handle* h, qfe_stats* out) {
     if (!h || !out) throw std::invalid_argument("null argument");
     *out = h->e.stats;
   });
+}
+
+int qfe_class_stats(const qfe_handle* h, int cap, int* cls, double* quartets, double* prim_quartets) {
+  if (!h) return -1;
+  const auto& e = h->e;
+  const int ncp = (int)e.cp_bra.size();
+  for (int cp = 0; cp < ncp && cp < cap; ++cp) {
+    const int b = e.cp_bra[cp], k = e.cp_ket[cp];
+    auto la = [](int c) {
+      int l = 0;
+      while ((l + 1) * (l + 2) / 2 <= c) ++l;
+      return l;
+    };
+    cls[4 * cp + 0] = la(b);
+    cls[4 * cp + 1] = b - la(b) * (la(b) + 1) / 2;
+    cls[4 * cp + 2] = la(k);
+    cls[4 * cp + 3] = k - la(k) * (la(k) + 1) / 2;
+    quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[cp];
+    prim_quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[ncp + cp];
+  }
+  return ncp;
 }
 
 int qfe_nbf(const qfe_handle* h) { return h ? h->e.nbf : -1; }

@@ -32,6 +32,17 @@ constexpr int chunk_size() {
   return ch;
 }
 
+// sum over the 64 lanes of a full wave (result valid on every lane)
+QFE_DI double wave_sum(double v) {
+  v += __shfl_xor(v, 32);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 1);
+  return v;
+}
+
 // ------------------------------------------------------------ J/K kernel
 template <class C, int Q0, int Q1>
 __global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, long long ntask,
@@ -51,6 +62,11 @@ __global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, 
 
   const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
                      (tk.x == tk.y ? 1.0 : 2.0);
+  // is the bra pair the same on every active lane of this wave?
+  const int lane = (int)(threadIdx.x & 63);
+  const unsigned long long active = __ballot(1);
+  const int bra0 = __shfl(tk.x, __ffsll((long long)active) - 1);
+  const bool uniform_bra = (active == ~0ull) && (__ballot(tk.x != bra0) == 0ull);
   const int i0 = bra.bfa, j0 = bra.bfb, k0 = ket.bfa, l0 = ket.bfb;
   const size_t n = (size_t)nbf;
   constexpr int MA0 = Q0 / C::NSB, MA1 = (Q1 - 1) / C::NSB;
@@ -92,7 +108,13 @@ __global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, 
         kjl[mb][d] = fma(Pik, x, kjl[mb][d]);
       });
     });
-    unsafeAtomicAdd(&J[ri + j0 + mb], jij);
+    if (uniform_bra) {
+      // every lane holds the same (i, j): one atomic per wave
+      const double tot = wave_sum(jij);
+      if (lane == 0) unsafeAtomicAdd(&J[ri + j0 + mb], tot);
+    } else {
+      unsafeAtomicAdd(&J[ri + j0 + mb], jij);
+    }
   });
   sfor<C::NSC>([&](auto c) {
     sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&J[(k0 + c) * n + l0 + d], jkl[c * C::NSD + d]); });

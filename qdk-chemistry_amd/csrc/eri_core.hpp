@@ -78,34 +78,117 @@ struct RysArgs {
   double tmax[9];
   double hx2[9][8];        // large-T limit: u_i = hx2/T
   double hw[9][8];         //               w_i = hw/sqrt(T)
+  const double* boys;      // Boys Taylor table [BOYS_NG][BOYS_W]: F_m(k * BOYS_DT)
 };
 constexpr int RYS_DEG = 11;
+constexpr double BOYS_DT = 0.05;
+constexpr double BOYS_TSW = 40.0;  // above: F_0 = sqrt(pi/T)/2 (erfc(sqrt 40) < 4e-19) and upward recursion
+constexpr int BOYS_NG = 802;
+constexpr int BOYS_W = 24;
+
+// 1/x and 1/sqrt(x) for positive normal x: hardware estimate plus two
+// Newton steps (quadratic convergence to full double precision).
+QFE_DI double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+QFE_DI double frsq(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  r = r * fma(-hx * r, r, 1.5);
+  return r * fma(-hx * r, r, 1.5);
+}
+
+// F_0..F_M at T.  Below BOYS_TSW: 7-term Taylor expansion about the nearest
+// grid point (|dT| <= 0.025, truncation < 2e-16 relative); for M <= 2 every
+// order is expanded directly (no exp), otherwise F_M is expanded and the
+// downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) fills the rest.  Above:
+// F_0 from the asymptotic form and the upward recursion (stable for T > M;
+// e^-T < 4e-18 is dropped).
+template <int M>
+QFE_DI void boys_fm(double T, const double* __restrict__ tab, double (&F)[M + 1]) {
+  if (T < BOYS_TSW) {
+    const int g = (int)(T * (1.0 / BOYS_DT) + 0.5);
+    const double d = (double)g * BOYS_DT - T;
+    const double* __restrict__ row = tab + (size_t)g * BOYS_W;
+    if constexpr (M <= 2) {
+      sfor<M + 1>([&](auto m) {
+        double s = row[m + 6];
+        sfor<6>([&](auto kk) {
+          constexpr int k = 5 - kk;
+          s = fma(s, d * (1.0 / (double)(k + 1)), row[m + k]);
+        });
+        F[m] = s;
+      });
+    } else {
+      const double e = exp(-T);
+      double s = row[M + 6];
+      sfor<6>([&](auto kk) {
+        constexpr int k = 5 - kk;
+        s = fma(s, d * (1.0 / (double)(k + 1)), row[M + k]);
+      });
+      F[M] = s;
+      sfor<M>([&](auto mm) {
+        constexpr int m = M - 1 - mm;
+        F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (double)(2 * m + 1));
+      });
+    }
+  } else {
+    const double it = frcp(T);
+    F[0] = 0.88622692545275801365 * frsq(T);  // sqrt(pi)/2 / sqrt(T)
+    sfor<M>([&](auto m) { F[m + 1] = (double)(2 * m + 1) * F[m] * (0.5 * it); });
+  }
+}
 
 template <int N>
 QFE_DI void rys_roots(double T, const RysArgs& ra, double (&u)[N], double (&w)[N]) {
-  if (T >= ra.tmax[N]) {
-    const double it = 1.0 / T;
-    const double is = sqrt(it);
-    sfor<N>([&](auto i) {
-      u[i] = ra.hx2[N][i] * it;
-      w[i] = ra.hw[N][i] * is;
-    });
+  if constexpr (N <= 2) {
+    // Gauss rule of the Boys measure from its moments F_0..F_{2N-1}
+    double F[2 * N];
+    boys_fm<2 * N - 1>(T, ra.boys, F);
+    if constexpr (N == 1) {
+      w[0] = F[0];
+      u[0] = F[1] * frcp(F[0]);
+    } else {
+      // monic pi_2(x) = x^2 + a x + b orthogonal to 1 and x
+      const double idet = frcp(F[0] * F[2] - F[1] * F[1]);
+      const double a = (F[1] * F[2] - F[0] * F[3]) * idet;
+      const double b = (F[1] * F[3] - F[2] * F[2]) * idet;
+      const double disc = a * a - 4.0 * b;
+      const double q = 0.5 * (disc * frsq(disc) - a);  // larger root (a < 0)
+      const double x1 = q, x0 = b * frcp(q);
+      const double w0 = (F[1] - x1 * F[0]) * frcp(x0 - x1);
+      u[0] = x0;
+      u[1] = x1;
+      w[0] = w0;
+      w[1] = F[0] - w0;
+    }
   } else {
-    const int k = (int)T;
-    const double x = T - ((double)k + 0.5);
-    const double* __restrict__ c = ra.coef[N] + (size_t)k * (2 * N) * (RYS_DEG + 1);
-    sfor<N>([&](auto i) {
-      const double* cu = c + i * (RYS_DEG + 1);
-      const double* cw = c + (N + i) * (RYS_DEG + 1);
-      double vu = cu[RYS_DEG], vw = cw[RYS_DEG];
+    if (T >= ra.tmax[N]) {
+      const double it = frcp(T);
+      const double is = frsq(T);
+      sfor<N>([&](auto i) {
+        u[i] = ra.hx2[N][i] * it;
+        w[i] = ra.hw[N][i] * is;
+      });
+    } else {
+      const int k = (int)T;
+      const double x = T - ((double)k + 0.5);
+      const double* __restrict__ c = ra.coef[N] + (size_t)k * (2 * N) * (RYS_DEG + 1);
+      sfor<N>([&](auto i) {
+        const double* cu = c + i * (RYS_DEG + 1);
+        const double* cw = c + (N + i) * (RYS_DEG + 1);
+        double vu = cu[RYS_DEG], vw = cw[RYS_DEG];
 #pragma unroll
-      for (int d = RYS_DEG - 1; d >= 0; --d) {
-        vu = fma(vu, x, cu[d]);
-        vw = fma(vw, x, cw[d]);
-      }
-      u[i] = vu;
-      w[i] = vw;
-    });
+        for (int d = RYS_DEG - 1; d >= 0; --d) {
+          vu = fma(vu, x, cu[d]);
+          vw = fma(vw, x, cw[d]);
+        }
+        u[i] = vu;
+        w[i] = vw;
+      });
+    }
   }
 }
 
@@ -208,14 +291,15 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket, const double4
       const double q = Q.x;
       const double QCx = Q.y - ket.A[0], QCy = Q.z - ket.A[1], QCz = Q.w - ket.A[2];
       const double PQx = P.y - Q.y, PQy = P.z - Q.z, PQz = P.w - Q.w;
-      const double rpq = 1.0 / (p + q);
+      const double rs = frsq(p + q);
+      const double rpq = rs * rs;
       const double rho = p * q * rpq;
       const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = TWO_PI_52 * Kb * Kk * sqrt(rpq);
+      const double pref = TWO_PI_52 * Kb * Kk * rs;
       double u[C::NR], w[C::NR];
       rys_roots<C::NR>(T, ra, u, w);
       const double qr = q * rpq, pr = p * rpq;
-      const double h2p = 0.5 / p, h2q = 0.5 / q;
+      const double h2p = 0.5 * frcp(p), h2q = 0.5 * frcp(q);
 #pragma unroll 1
       for (int r = 0; r < C::NR; ++r) {
         const double ur = u[r];

@@ -51,10 +51,9 @@ __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __r
 
 // ------------------------------------------------------------ task list
 // work item = (bra pair p, ket bucket b); one wavefront scans bucket b.
-__device__ __forceinline__ bool quartet_passes(const PairHdr& P1, double Q1, const PairHdr& P2,
-                                               double Q2, const double* __restrict__ Pn, int nsh,
-                                               double thr) {
-  const size_t a = P1.sa, b = P1.sb, c = P2.sa, d = P2.sb;
+__device__ __forceinline__ bool quartet_passes(int2 P1, double Q1, int2 P2, double Q2,
+                                               const double* __restrict__ Pn, int nsh, double thr) {
+  const size_t a = P1.x, b = P1.y, c = P2.x, d = P2.y;
   double m = Pn[a * nsh + b];
   m = fmax(m, Pn[c * nsh + d]);
   m = fmax(m, Pn[a * nsh + c]);
@@ -68,7 +67,7 @@ template <bool WRITE>
 __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ items, long long nitems,
                                                    const int2* __restrict__ buckets,
                                                    const int* __restrict__ pair_bucket,
-                                                   const PairHdr* __restrict__ pairs,
+                                                   const int2* __restrict__ psh,
                                                    const double* __restrict__ Q,
                                                    const double* __restrict__ Pn, int nsh,
                                                    const double* __restrict__ pmax_ptr, double thr,
@@ -78,13 +77,14 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
   const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (item >= nitems) return;
+  if (WRITE && counts[item] == 0) return;
   const int2 it = items[item];
   const int p = it.x;
   const int2 bk = buckets[it.y];
   int end = bk.y;
   if (pair_bucket[p] == it.y) end = p + 1;  // same bucket: q <= p
   const double Qp = Q[p];
-  const PairHdr Pp = pairs[p];
+  const int2 Pp = psh[p];
   const double cut = thr / (*pmax_ptr);
   long long base = WRITE ? offsets[item] : 0;
   int count = 0;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
     bool ok = false;
     if (q < end) {
       const double Qq = Q[q];
-      ok = quartet_passes(Pp, Qp, pairs[q], Qq, Pn, nsh, thr);
+      ok = quartet_passes(Pp, Qp, psh[q], Qq, Pn, nsh, thr);
     }
     const unsigned long long m = __ballot(ok);
     if (WRITE && ok) {
@@ -105,6 +105,38 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
     count += __popcll(m);
   }
   if (!WRITE && lane == 0) counts[item] = count;
+}
+
+// per class pair: surviving shell quartets and their primitive quartets
+// (block-level LDS reduction, one global atomic per block and class pair)
+constexpr int kMaxCP = 64;
+__global__ __launch_bounds__(256) void k_class_stats(const int* __restrict__ counts, const int2* __restrict__ items,
+                                                     long long nitems, const int* __restrict__ item_cp,
+                                                     const int2* __restrict__ buckets,
+                                                     const PairHdr* __restrict__ pairs, int ncp,
+                                                     double* __restrict__ q_out, double* __restrict__ p_out) {
+  __shared__ double sq[kMaxCP], sp[kMaxCP];
+  for (int k = threadIdx.x; k < kMaxCP; k += blockDim.x) {
+    sq[k] = 0.0;
+    sp[k] = 0.0;
+  }
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nitems) {
+    const int c = counts[i];
+    if (c != 0) {
+      const int2 it = items[i];
+      const double np = (double)pairs[it.x].nprim * (double)pairs[buckets[it.y].x].nprim;
+      atomicAdd(&sq[item_cp[i]], (double)c);
+      atomicAdd(&sp[item_cp[i]], (double)c * np);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ncp && k < kMaxCP; k += blockDim.x)
+    if (sq[k] != 0.0) {
+      unsafeAtomicAdd(&q_out[k], sq[k]);
+      unsafeAtomicAdd(&p_out[k], sp[k]);
+    }
 }
 
 __global__ void k_gather_offsets(const long long* __restrict__ offsets, const long long* __restrict__ idx,
@@ -144,7 +176,7 @@ void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* s
 }
 
 void launch_task_count(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
-                       const PairHdr* pairs, const double* Q, const double* Pn, int nsh,
+                       const int2* pairs, const double* Q, const double* Pn, int nsh,
                        const double* pmax, double thr, int* counts, hipStream_t s) {
   if (nitems <= 0) return;
   const long long threads = nitems * 64;
@@ -155,14 +187,14 @@ void launch_task_count(const int2* items, long long nitems, const int2* buckets,
 }
 
 void launch_task_write(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
-                       const PairHdr* pairs, const double* Q, const double* Pn, int nsh,
-                       const double* pmax, double thr, const long long* offsets, int2* tasks,
-                       hipStream_t s) {
+                       const int2* pairs, const double* Q, const double* Pn, int nsh,
+                       const double* pmax, double thr, const int* counts, const long long* offsets,
+                       int2* tasks, hipStream_t s) {
   if (nitems <= 0) return;
   const long long threads = nitems * 64;
   const long long grid = (threads + 255) / 256;
   hipLaunchKernelGGL((k_task_scan<true>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
-                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, (int*)nullptr, offsets, tasks);
+                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, (int*)counts, offsets, tasks);
 }
 
 size_t scan_temp_bytes(long long n) {
@@ -179,6 +211,14 @@ void launch_exclusive_scan(void* temp, size_t temp_bytes, const int* in, long lo
 void launch_gather_offsets(const long long* offsets, const long long* idx, int n, long long* out,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_gather_offsets, dim3((n + 63) / 64), dim3(64), 0, s, offsets, idx, n, out);
+}
+
+void launch_class_stats(const int* counts, const int2* items, long long nitems, const int* item_cp,
+                        const int2* buckets, const PairHdr* pairs, int ncp, double* q_out, double* p_out,
+                        hipStream_t s) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_class_stats, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0, s, counts, items,
+                     nitems, item_cp, buckets, pairs, ncp, q_out, p_out);
 }
 
 void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {

@@ -38,6 +38,67 @@ class SCFConfig:
     eri: ERIConfig = field(default_factory=ERIConfig)
 
 
+def _fractional_occupations(eps, nelec):
+    """Fill orbitals by energy; a degenerate group at the Fermi level shares
+    the remaining electrons equally (spherical average of an open shell)."""
+    occ = np.zeros_like(eps)
+    left = float(nelec)
+    i = 0
+    n = len(eps)
+    while left > 1e-12 and i < n:
+        j = i
+        while j + 1 < n and abs(eps[j + 1] - eps[i]) < 1e-4:
+            j += 1
+        g = j - i + 1
+        take = min(left, 2.0 * g)
+        occ[i:j + 1] = take / g
+        left -= take
+        i = j + 1
+    return occ
+
+
+def atom_guess(structure, basis_name, pure=True, eri_cfg=None, iterations=30):
+    """Superposition of spherically averaged atomic densities (the role of
+    atom_guess, scf/src/scf/guess.cpp:23-101): one fractional-occupation SCF
+    per element with the same engine, placed block-diagonally in AO order."""
+    from .basis import load_basis
+    from .molecule import Structure
+    mol_basis = load_basis(basis_name, structure.numbers, structure.coords, pure=pure)
+    P = np.zeros((mol_basis.nbf, mol_basis.nbf))
+    dens = {}
+    for z in sorted(set(int(x) for x in structure.numbers)):
+        at = Structure(np.array([z]), np.zeros((1, 3)))
+        b = load_basis(basis_name, at.numbers, at.coords, pure=pure)
+        eri = create(b, at, eri_cfg or ERIConfig())
+        S = one_body(b, at, 0)
+        H = one_body(b, at, 1) + one_body(b, at, 2)
+        X = orthogonalizer(S)
+        e, Cp = np.linalg.eigh(X.T @ H @ X)
+        C = X @ Cp
+        Pa = (C * _fractional_occupations(e, z)) @ C.T
+        for _ in range(iterations):
+            J, K = eri.build_JK(Pa)
+            F = H + J - 0.5 * K
+            e, Cp = np.linalg.eigh(X.T @ F @ X)
+            C = X @ Cp
+            Pn = (C * _fractional_occupations(e, z)) @ C.T
+            done = np.max(np.abs(Pn - Pa)) < 1e-7
+            Pa = 0.5 * (Pa + Pn) if not done else Pn
+            if done:
+                break
+        dens[z] = Pa
+    # scatter: shells of each atom keep their relative order in the molecule
+    offs = mol_basis.shell_offsets
+    for ia, z in enumerate(structure.numbers):
+        idx = []
+        for s, sh in enumerate(mol_basis.shells):
+            if sh.atom == ia:
+                idx.extend(range(offs[s], offs[s + 1]))
+        idx = np.array(idx)
+        P[np.ix_(idx, idx)] = dens[int(z)]
+    return P
+
+
 def orthogonalizer(S, lindep=-1.0):
     s, U = np.linalg.eigh(S)
     thr = lindep if lindep >= 0 else s.max() / 1e9
@@ -81,10 +142,12 @@ class RHF:
     def energy(self, P, F):
         return self.E_nuc + float(np.sum(P * self.H)) + 0.5 * float(np.sum(P * (F - self.H)))
 
-    def run(self, P0=None, verbose=False):
+    def run(self, P0=None, verbose=False, guess="atom"):
         cfg = self.cfg
         n = self.basis.nbf
-        if P0 is None:
+        if P0 is None and guess == "atom":
+            P = atom_guess(self.structure, self.basis.name, self.basis.pure, self.cfg.eri)
+        elif P0 is None:
             e, C = self._diag(self.H)
             P = self._density(C)
         else:

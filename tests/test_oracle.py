@@ -1,0 +1,153 @@
+"""Pins the oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each check is chosen so a plausible mistake (dropped term, wrong sign or
+index, transposed operand) fails it: 40-digit Boys values, the (ss|ss) closed
+form, ERI permutation symmetry / translation invariance / the Schwarz
+inequality, direct J/K vs the full-tensor contraction (the reference's own
+EriMethodSetting check, cpp/tests/test_scf.cpp:594-637), and SCF energies
+against the reference's fixtures.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import random_density
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_boys_vs_mpmath():
+    from oracle.oracle import boys
+    worst = 0.0
+    for m, T, v in json.load(open(os.path.join(GOLD, "boys.json"))):
+        got = boys(m, T)[m]
+        ref = float(v)
+        worst = max(worst, abs(got - ref) / ref)
+    assert worst < 5e-14
+
+
+def test_boys_closed_form_f0():
+    from oracle.oracle import boys
+    for T in (0.01, 0.7, 3.3, 15.0, 44.0, 90.0):
+        ref = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        assert abs(boys(0, T)[0] - ref) <= 2e-15 * ref
+
+
+def _s_basis(exps, centers):
+    """Uncontracted normalised s shells (one per center)."""
+    from qdk_chemistry_amd.basis import BasisSet, Shell, normalize_psi4
+    shells = [Shell(i, 0, np.array([a]), normalize_psi4(0, np.array([a]), np.array([1.0])), np.array(c, float))
+              for i, (a, c) in enumerate(zip(exps, centers))]
+    return BasisSet("s", shells, True, len(shells))
+
+
+def test_ssss_closed_form():
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.molecule import Structure
+    exps = [1.3, 0.4, 2.2, 0.7]
+    cen = [[0, 0, 0], [0.5, -0.3, 0.9], [1.9, 0.2, -0.4], [-0.6, 1.1, 0.3]]
+    b = _s_basis(exps, cen)
+    mol = Structure(np.ones(4, dtype=int), np.array(cen, float))
+    o = OracleERI(b, mol)
+    v = o.quartet(0, 1, 2, 3)[0, 0, 0, 0]
+    a, bb, c, d = exps
+    A, B, C, D = (np.array(x, float) for x in cen)
+    p, q = a + bb, c + d
+    P, Q = (a * A + bb * B) / p, (c * C + d * D) / q
+    N = [(2 * x / math.pi) ** 0.75 for x in exps]
+    Kab = math.exp(-a * bb / p * np.sum((A - B) ** 2))
+    Kcd = math.exp(-c * d / q * np.sum((C - D) ** 2))
+    T = p * q / (p + q) * np.sum((P - Q) ** 2)
+    F0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    ref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * Kab * Kcd * F0 * np.prod(N)
+    assert abs(v - ref) < 1e-14 * abs(ref)
+
+
+@pytest.fixture(scope="module")
+def water_oracle(water):
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    b = load_basis("cc-pvdz", water.numbers, water.coords)
+    return b, OracleERI(b, water)
+
+
+def test_eri_permutation_symmetry(water_oracle):
+    b, o = water_oracle
+    rng = np.random.Generator(np.random.PCG64(3))
+    for _ in range(12):
+        s = rng.integers(0, b.nsh, 4)
+        v = o.quartet(*s)
+        assert np.allclose(o.quartet(s[1], s[0], s[2], s[3]), v.transpose(1, 0, 2, 3), atol=1e-13)
+        assert np.allclose(o.quartet(s[0], s[1], s[3], s[2]), v.transpose(0, 1, 3, 2), atol=1e-13)
+        assert np.allclose(o.quartet(s[2], s[3], s[0], s[1]), v.transpose(2, 3, 0, 1), atol=1e-13)
+
+
+def test_eri_translation_invariance(water):
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.molecule import Structure
+    b1 = load_basis("cc-pvdz", water.numbers, water.coords)
+    shifted = Structure(water.numbers, water.coords + np.array([0.7, -1.3, 2.1]))
+    b2 = load_basis("cc-pvdz", shifted.numbers, shifted.coords)
+    o1, o2 = OracleERI(b1, water), OracleERI(b2, shifted)
+    for s in ((11, 9, 7, 0), (11, 11, 10, 8), (3, 7, 11, 2)):
+        assert np.allclose(o1.quartet(*s), o2.quartet(*s), atol=1e-12)
+
+
+def test_schwarz_inequality(water_oracle):
+    b, o = water_oracle
+    Q = o.schwarz()
+    rng = np.random.Generator(np.random.PCG64(9))
+    for _ in range(30):
+        s = rng.integers(0, b.nsh, 4)
+        v = np.max(np.abs(o.quartet(*s)))
+        assert v <= Q[s[0], s[1]] * Q[s[2], s[3]] * (1 + 1e-12)
+
+
+def test_direct_equals_incore(water_oracle):
+    """libint2_direct build_JK semantics == full-tensor J/K (incore_impl.cpp:159-197)."""
+    b, o = water_oracle
+    o2 = type(o)(b, o.structure, eri_threshold=0.0, shell_pair_threshold=0.0)
+    P = random_density(b.nbf, 4)
+    J, K = o2.build_JK(P)
+    Ji, Ki = o2.build_JK_incore(P)
+    assert np.max(np.abs(J - Ji)) < 1e-12
+    assert np.max(np.abs(K - Ki)) < 1e-12
+
+
+def test_nan_density_raises(water_oracle):
+    b, o = water_oracle
+    P = np.zeros((b.nbf, b.nbf))
+    P[3, 4] = np.nan
+    with pytest.raises(RuntimeError):
+        o.build_JK(P)
+
+
+CRAWFORD = [[0.0, -0.0757918436, 0.0], [0.866811829, 0.6014357793, 0.0], [-0.866811829, 0.6014357793, 0.0]]
+
+
+@pytest.mark.parametrize("case", [
+    # examples/data/water.structure.xyz, python/tests/test_sample_workflow_sci.py:165,201-204
+    ("cc-pvdz", "example", -76.02286895, 1e-8),
+    # cpp/tests/test_scf.cpp:136-150 (Crawford geometry, cpp/tests/ut_common.hpp:244-260)
+    ("def2-svp", "crawford", -75.9229032345009, 1e-8),
+    # scf/tests/data/h2o.json RHF/HF/6-31g*/pure, geometry scf/tests/test_common.h:11-32
+    ("6-31g*", "h2o_json", -76.00450217800814, 1e-8),
+])
+def test_oracle_scf_energy(water, case):
+    from oracle.oracle import rhf
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.molecule import Structure
+    basis, geo, ref, tol = case
+    if geo == "example":
+        mol = water
+    elif geo == "crawford":
+        mol = Structure.from_angstrom(["O", "H", "H"], CRAWFORD)
+    else:
+        mol = Structure(np.array([1, 1, 8]), np.array([[0.0, 0.49, -0.79], [0.0, 0.49, 0.79], [0.0, -0.12, 0.0]]) / 0.52917721092)
+    b = load_basis(basis, mol.numbers, mol.coords)
+    r = rhf(mol, b)
+    assert abs(r["energy"] - ref) < tol
