@@ -1,0 +1,237 @@
+"""Fock-build benchmark: (H2O)64 / cc-pVDZ (1536 basis functions) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--system h2o64|h2o16]
+                    [--no-scf] [--no-cpu] [--no-mixed]
+
+One step is one full J/K build (screened task list + ERI class kernels +
+digestion + symmetrisation) of the seeded (H2O)64 cluster at the converged
+RHF density, with the density and J/K resident in HBM.  Before the timed
+region the RHF is converged in FP64 (plugin defaults: convergence 1e-7,
+eri_threshold 1e-12, shell_pair_threshold 1e-12); with mixed precision the
+same SCF is repeated with FP32 for quartets whose Schwarz bound is below 1e-6
+and the energy difference is reported.  The CPU baseline is the oracle (a C
+OpenMP restatement of the reference's LIBINT2_DIRECT loop with its own
+Obara-Saika integrals, not Libint2) on a strided sample of bra shells of the
+same build.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "qdk-chemistry_amd"))
+
+FP64_PEAK = 78.6e12  # MI355X dense FP64 vector, FLOP/s
+FP32_PEAK = 157.3e12  # MI355X FP32 vector, FLOP/s
+METRIC = "Fock-build s and ERI quartets/s, (H2O)64/cc-pVDZ, mixed vs FP64 at 1 µEh"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def system(name):
+    from qdk_chemistry_amd.molecule import water_cluster
+    if name == "h2o64":
+        return water_cluster(64, 20261264), "cc-pvdz", "(H2O)64/cc-pVDZ seed 20261264"
+    if name == "h2o16":
+        return water_cluster(16, 20261202), "cc-pvdz", "(H2O)16/cc-pVDZ seed 20261202"
+    raise SystemExit(f"unknown system {name}")
+
+
+def timed_builds(eng, P, steps, warmup):
+    """K builds on HBM-resident P/J/K; per-class counts and kernel times
+    (HIP events recorded on the engine stream) for both precisions."""
+    import numpy as np
+    import torch
+    dP = torch.from_numpy(np.ascontiguousarray(P)).to("cuda:0")
+    dJ = torch.empty_like(dP)
+    dK = torch.empty_like(dP)
+    for _ in range(warmup):
+        eng.build_JK_device(dP, dJ, dK)
+    torch.cuda.synchronize()
+    per_class = {}
+    quartets = quartets32 = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.build_JK_device(dP, dJ, dK)
+        for prec in (0, 1):
+            for cls, nq, npq, ms in eng.class_stats(prec):
+                if nq > 0:
+                    acc = per_class.setdefault((prec, cls), [0.0, 0.0, 0.0, 0])
+                    acc[0] += nq
+                    acc[1] += npq
+                    acc[2] += ms
+                    acc[3] += 1
+        st = eng.stats()
+        quartets += st["quartets"]
+        quartets32 += st["quartets_fp32"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(seconds=dt, quartets=quartets, quartets_fp32=quartets32, per_class=per_class, stats=eng.stats(),
+                J=dJ, K=dK)
+
+
+def summarize(run, steps):
+    from qdk_chemistry_amd.flops import class_flops
+    f64 = f32 = 0.0
+    dom = None
+    for (prec, cls), (nq, npq, ms, n) in run["per_class"].items():
+        fp, fc = class_flops(*cls)
+        f = npq * fp + nq * fc
+        if prec == 0:
+            f64 += f
+            if dom is None or ms > dom[3]:
+                dom = (cls, f / n, nq / n, ms / n)
+        else:
+            f32 += f
+    eri_ms = sum(v[2] for v in run["per_class"].values()) / steps
+    ms_step = 1e3 * run["seconds"] / steps
+    t_peak = (f64 / FP64_PEAK + f32 / FP32_PEAK) / steps
+    return dict(ms_per_build=ms_step, quartets_per_s=run["quartets"] / run["seconds"],
+                quartets_per_build=run["quartets"] / steps, fp32_fraction=run["quartets_fp32"] / max(run["quartets"], 1),
+                eri_kernels_ms=eri_ms, screen_ms=run["stats"]["ms_screen"],
+                flops_fp64_per_build=f64 / steps, flops_fp32_per_build=f32 / steps,
+                peak_fraction_eri=t_peak / (eri_ms * 1e-3) if eri_ms > 0 else 0.0,
+                dominant=dom)
+
+
+def cpu_baseline(basis, mol, P, stride):
+    from oracle.oracle import OracleERI
+    threads = min(16, os.cpu_count() or 1)
+    o = OracleERI(basis, mol, nthreads=threads)
+    t0 = time.perf_counter()
+    o.build_JK_sample(P, stride, 0)
+    dt = time.perf_counter() - t0
+    q = o.last_stats["quartets"]
+    return dict(value=q / dt, unit="quartets/s", cores=threads, kind="port",
+                sample=f"oracle build_JK (C/OpenMP restatement of LIBINT2_DIRECT, Obara-Saika integrals, "
+                       f"not Libint2) over bra shells s1 % {stride} == 0 of the same build: "
+                       f"{int(q)} quartets in {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--system", default="h2o64")
+    ap.add_argument("--no-scf", action="store_true", help="time at the atomic-guess density")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=24)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-device "
+              f"in this build.  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("no GPU visible")
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.flops import class_flops
+    from qdk_chemistry_amd.scf import RHF, SCFConfig, atom_guess
+
+    mol, basis_name, desc = system(args.system)
+    basis = load_basis(basis_name, mol.numbers, mol.coords)
+    cfg = ERIConfig(eri_threshold=1e-12, shell_pair_threshold=1e-12, profile=True)
+    t = time.perf_counter()
+    eng = create(basis, mol, cfg)
+    log(f"engine: {desc}, nbf={basis.nbf}, setup {time.perf_counter() - t:.2f}s, "
+        f"pairs={eng.stats()['significant_pairs']}")
+
+    result = {}
+    if args.no_scf:
+        P = atom_guess(mol, basis_name)
+        density = "atomic guess"
+    else:
+        t = time.perf_counter()
+        scf = RHF(mol, basis, SCFConfig(eri=cfg), eri=eng)
+        r = scf.run()
+        P = r["P"]
+        density = "converged FP64 RHF"
+        result["scf_fp64"] = dict(energy=r["energy"], iterations=r["iterations"],
+                                  fock_build_s=float(np.sum(scf.build_times)), wall_s=time.perf_counter() - t)
+        log(f"FP64 SCF: E={r['energy']:.10f} in {r['iterations']} iterations, "
+            f"Fock builds {np.sum(scf.build_times):.2f}s")
+
+    run64 = timed_builds(eng, P, args.steps, args.warmup)
+    s64 = summarize(run64, args.steps)
+    cls, f_launch, nq_launch, ms_launch = s64["dominant"]
+    achieved = f_launch / (ms_launch * 1e-3)
+    log(f"FP64 timed: {args.steps} builds, {s64['ms_per_build']:.2f} ms/build, {s64['quartets_per_s']:.4g} quartets/s, "
+        f"ERI kernels {s64['eri_kernels_ms']:.2f} ms ({s64['peak_fraction_eri']:.3f} of peak), screen "
+        f"{s64['screen_ms']:.2f} ms; dominant class {cls}: {ms_launch:.2f} ms, {achieved / 1e12:.2f} TFLOP/s")
+    headline = s64
+    path = "fp64"
+    mixed = None
+    if not args.no_mixed:
+        eng.set_precision(True, 1e-6)
+        run32 = timed_builds(eng, P, args.steps, args.warmup)
+        eng.set_precision(False)
+        s32 = summarize(run32, args.steps)
+        import numpy as np
+        Jm, Km = run32["J"].cpu().numpy(), run32["K"].cpu().numpy()
+        J6, K6 = run64["J"].cpu().numpy(), run64["K"].cpu().numpy()
+        mixed = dict(s32)
+        mixed.pop("dominant")
+        mixed["max_abs_dF_vs_fp64"] = float(np.max(np.abs((Jm - 0.5 * Km) - (J6 - 0.5 * K6))))
+        mixed["speedup_vs_fp64"] = s64["ms_per_build"] / s32["ms_per_build"]
+        log(f"mixed timed: {s32['ms_per_build']:.2f} ms/build ({mixed['speedup_vs_fp64']:.2f}x), "
+            f"FP32 share {s32['fp32_fraction']:.3f}, max|dF| {mixed['max_abs_dF_vs_fp64']:.2e}")
+        if not args.no_scf:
+            t = time.perf_counter()
+            eng.set_precision(True, 1e-6)
+            scf_m = RHF(mol, basis, SCFConfig(eri=cfg), eri=eng)
+            rm = scf_m.run()
+            eng.set_precision(False)
+            dE = rm["energy"] - result["scf_fp64"]["energy"]
+            mixed["scf"] = dict(energy=rm["energy"], dE_vs_fp64=dE, iterations=rm["iterations"],
+                                mixed_builds=rm["mixed_builds"], fock_build_s=float(np.sum(scf_m.build_times)),
+                                wall_s=time.perf_counter() - t)
+            log(f"mixed SCF: E={rm['energy']:.10f} dE={dE:+.2e} ({rm['iterations']} iterations, "
+                f"{rm['mixed_builds']} mixed builds, Fock builds {np.sum(scf_m.build_times):.2f}s)")
+            ok = abs(dE) <= 1e-6
+        else:
+            ok = True
+        if ok and s32["ms_per_build"] < s64["ms_per_build"]:
+            headline, path = s32, "mixed"
+    out = {
+        "metric": METRIC,
+        "value": headline["quartets_per_s"],
+        "unit": "quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": headline["ms_per_build"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if path == "fp64" else "f64+f32",
+        "data": f"synthetic seeded geometry, {density} density",
+        "config": {"workload": f"RHF J/K build, {desc}, nbf={basis.nbf}, eri_threshold 1e-12",
+                   "path": path, "fock_build_s": headline["ms_per_build"] / 1e3,
+                   "fp64": {k: v for k, v in s64.items() if k != "dominant"}, "mixed": mixed},
+        "roofline": {"bound": "fp64-valu", "kernel": "k_eri_jk class (%d %d|%d %d), FP64" % cls,
+                     "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK, "traffic": None,
+                     "flops_per_launch": f_launch, "ms_per_launch": ms_launch},
+    }
+    out["config"].update(result)
+    if not args.no_cpu:
+        cb = cpu_baseline(basis, mol, P, args.cpu_stride)
+        out["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb['value']:.4g} quartets/s on {cb['cores']} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

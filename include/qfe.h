@@ -63,6 +63,7 @@ typedef struct {
   int mix_precision;           /* 1: FP32 for quartets whose Schwarz bound < mix_threshold */
   double mix_threshold;        /* 1e-6 */
   int verbose;
+  int profile;                 /* 1: time every class kernel group with HIP events */
 } qfe_config;
 
 /* Per-build statistics (the last build). */
@@ -106,13 +107,23 @@ int qfe_one_body(const qfe_basis* basis, int op, double* out);
 /* Gradient of sum_mn W_mn d(O)_mn / dR for op 0/1/2 (3*natoms, atom-major). */
 int qfe_one_body_grad(const qfe_basis* basis, int op, const double* W, double* grad);
 
+/* Switch the precision policy of subsequent builds: mix != 0 evaluates
+ * quartets with Q_ab Q_cd < threshold in FP32 (ERIConfig::is_mix_precision,
+ * core/scf.h:71-93, which the reference declares but no shipped engine reads);
+ * mix == 0 is all-FP64.  The SCF driver uses it for FP64 final iterations. */
+int qfe_set_precision(qfe_handle* h, int mix, double threshold);
+
 int qfe_get_stats(const qfe_handle* h, qfe_stats* out);
 
-/* Per-class census of the last build: for class pair i < min(cap, n):
+/* Per-class census of the last build for precision 0 (FP64 tasks) or 1
+ * (FP32 tasks of the mixed-precision path): for class pair i < min(cap, n):
  * cls[4i..4i+3] = (la, lb, lc, ld) as evaluated, quartets[i] and
- * prim_quartets[i] screened counts.  Returns n (number of class pairs), or
- * -1 for a null handle.  No reference counterpart: instrumentation. */
-int qfe_class_stats(const qfe_handle* h, int cap, int* cls, double* quartets, double* prim_quartets);
+ * prim_quartets[i] screened counts, ms[i] the kernel time of that class
+ * (HIP events, only with qfe_config.profile; else 0).  Returns n (number of
+ * class pairs), or -1 for a null handle.  No reference counterpart:
+ * instrumentation. */
+int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, double* quartets,
+                    double* prim_quartets, double* ms);
 int qfe_nbf(const qfe_handle* h);
 void qfe_destroy(qfe_handle* h);
 const char* qfe_last_error(void);

@@ -948,6 +948,14 @@ static inline double max6(double a, double b, double c, double d, double e, doub
  * stats (optional, length >= 4): [0] surviving shell quartets,
  * [1] primitive quartets of those, [2] function quartets, [3] loop quartets.
  */
+static int g_row_stride = 1, g_row_offset = 0;
+/* Restrict build_JK to bra shells s1 with s1 % stride == offset (a bounded
+ * sample of the same loop, for CPU timing); stride 1 = the full build. */
+void oracle_set_row_sample(int stride, int offset) {
+  g_row_stride = stride > 0 ? stride : 1;
+  g_row_offset = offset;
+}
+
 int oracle_build_jk(const oracle_t *h, const double *P, double *J, double *K, double alpha,
                     double beta, int ndm, double *stats) {
   const int nsh = h->nsh, n = h->nbf;
@@ -985,6 +993,7 @@ int oracle_build_jk(const oracle_t *h, const double *P, double *J, double *K, do
     double *Kt = K ? Kl + (size_t)tid * msz : NULL;
     size_t s1234 = 0;
     for (int s1 = 0; s1 < nsh; ++s1) {
+      if (s1 % g_row_stride != g_row_offset) continue;
       const int bf1 = h->bf0[s1], n1 = h->bf0[s1 + 1] - bf1;
       for (int x12 = h->splist_off[s1]; x12 < h->splist_off[s1 + 1]; ++x12) {
         const int s2 = h->splist[x12];

@@ -59,6 +59,7 @@ def lib():
         L.oracle_boys.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
         L.oracle_sph_row.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_set_row_sample.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -139,6 +140,14 @@ class OracleERI:
             raise RuntimeError("Density matrix contains NaN/Inf values.")
         self.last_stats = dict(quartets=st[0], prim_quartets=st[1], func_quartets=st[2], loop_quartets=st[3])
         return J, K
+
+    def build_JK_sample(self, P, stride, offset=0):
+        """build_JK over bra shells s1 % stride == offset only (timing sample)."""
+        lib().oracle_set_row_sample(stride, offset)
+        try:
+            return self.build_JK(P)
+        finally:
+            lib().oracle_set_row_sample(1, 0)
 
     def census(self, P):
         P = np.ascontiguousarray(P, dtype=np.float64)

@@ -174,7 +174,10 @@ struct Engine {
   bool mix = false;
   double mix_thr = 1e-6;
   int verbose = 0;
+  bool profile = false;
   int nbf = 0, nsh = 0, natoms = 0;
+  std::vector<hipEvent_t> cls_ev;    // [2 precisions][ncp][2] when profiling
+  std::vector<double> cls_ms;        // [2][ncp]
 
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {};
@@ -194,12 +197,14 @@ struct Engine {
   DevBuf<int2> psh_d;  // (sa, sb) per sorted pair, for the screening scan
   DevBuf<double4> ppos_d;
   DevBuf<double> pk_d;
+  DevBuf<float4> ppos32_d;  // single-precision copies for the FP32 tasks
+  DevBuf<float> pk32_d;
   DevBuf<double> Q_d;
   DevBuf<int2> buckets_d;
   DevBuf<int> pair_bucket_d;
   DevBuf<int2> items_d;
   DevBuf<int> item_cp_d;
-  DevBuf<double> cstat_d;           // [2][ncp]: quartets, primitive quartets
+  DevBuf<double> cstat_d;           // [4][ncp]: quartets, prim quartets (FP64), then (FP32)
   std::vector<double> cstat_h;
   DevBuf<int> counts_d;
   DevBuf<long long> offsets_d;
@@ -208,17 +213,23 @@ struct Engine {
   size_t scan_tmp_bytes = 0;
   DevBuf<int2> tasks_d;
   DevBuf<int> sh_bf0_d, sh_nf_d;
-  DevBuf<double> Pn_d, pmax_d;
+  DevBuf<float> Pn_d;
+  DevBuf<double> Pnd_d, pmax_d;
   DevBuf<int> bad_d;
   DevBuf<double> P_d, J_d, K_d;
   std::vector<DevBuf<double>> rys_d;
   DevBuf<double> boys_d;
+  std::vector<DevBuf<float>> rys32_d;
+  DevBuf<float> boys32_d;
   RysArgs ra{};
+  RysArgs32 ra32{};
 
   qfe_stats stats{};
 
   ~Engine() {
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : cls_ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -247,9 +258,29 @@ struct Engine {
     return t;
   }
 
+  static std::vector<float> to_float(const std::vector<double>& v) {
+    std::vector<float> f(v.size());
+    for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
+    return f;
+  }
+
   void setup_rys() {
     const RysTableHost& t = rys_table();
-    boys_d.upload(boys_table(), stream);
+    const std::vector<double> bt = boys_table();
+    boys_d.upload(bt, stream);
+    boys32_d.upload(to_float(bt), stream);
+    rys32_d.resize(t.nmax + 1);
+    std::memset(&ra32, 0, sizeof(ra32));
+    for (int n = 1; n <= t.nmax && n <= 8; ++n) {
+      rys32_d[n].upload(to_float(t.coef[n]), stream);
+      ra32.coef[n] = rys32_d[n].p;
+      ra32.tmax[n] = (float)t.tmax[n];
+      for (int i = 0; i < n; ++i) {
+        ra32.hx2[n][i] = (float)t.hx2[n][i];
+        ra32.hw[n][i] = (float)t.hw[n][i];
+      }
+    }
+    ra32.boys = boys32_d.p;
     rys_d.resize(t.nmax + 1);
     std::memset(&ra, 0, sizeof(ra));
     for (int n = 1; n <= t.nmax && n <= 8; ++n) {
@@ -412,6 +443,13 @@ struct Engine {
     psh_d.upload(psh, stream);
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
+    {
+      std::vector<float4> p32(ppos2.size());
+      for (size_t i = 0; i < ppos2.size(); ++i)
+        p32[i] = make_float4((float)ppos2[i].x, (float)ppos2[i].y, (float)ppos2[i].z, (float)ppos2[i].w);
+      ppos32_d.upload(p32, stream);
+      pk32_d.upload(to_float(pk2), stream);
+    }
     Q_d.upload(Q_h, stream);
     buckets_d.upload(buckets_h, stream);
     pair_bucket_d.upload(pair_bucket_h, stream);
@@ -420,18 +458,22 @@ struct Engine {
     for (int cp = 0; cp < ncp; ++cp)
       for (int i = cp_item_begin[cp]; i < cp_item_begin[cp + 1]; ++i) item_cp[i] = cp;
     item_cp_d.upload(item_cp, stream);
-    cstat_d.alloc(2 * (size_t)ncp);
-    cstat_h.assign(2 * (size_t)ncp, 0.0);
+    cstat_d.alloc(4 * (size_t)ncp);
+    cstat_h.assign(4 * (size_t)ncp, 0.0);
     const long long nitems = (long long)items_h.size();
-    counts_d.alloc(nitems + 1);
-    HIP_CHECK(hipMemsetAsync(counts_d.p, 0, sizeof(int) * (nitems + 1), stream));
-    offsets_d.alloc(nitems + 1);
-    scan_tmp_bytes = scan_temp_bytes(nitems + 1);
+    counts_d.alloc(2 * nitems + 1);
+    HIP_CHECK(hipMemsetAsync(counts_d.p, 0, sizeof(int) * (2 * nitems + 1), stream));
+    offsets_d.alloc(2 * nitems + 1);
+    scan_tmp_bytes = scan_temp_bytes(2 * nitems + 1);
     scan_tmp_d.alloc(scan_tmp_bytes);
-    std::vector<long long> bidx(ncp + 1);
-    for (int cp = 0; cp <= ncp; ++cp) bidx[cp] = cp_item_begin[cp];
+    // class-pair boundaries in the scanned counts: FP64 part, then FP32 part
+    std::vector<long long> bidx(2 * (ncp + 1));
+    for (int cp = 0; cp <= ncp; ++cp) {
+      bidx[cp] = cp_item_begin[cp];
+      bidx[ncp + 1 + cp] = nitems + cp_item_begin[cp];
+    }
     bidx_d.upload(bidx, stream);
-    bval_d.alloc(ncp + 1);
+    bval_d.alloc(2 * (ncp + 1));
     HIP_CHECK(hipStreamSynchronize(stream));
     if (verbose)
       fprintf(stderr, "[qfe] nsh=%d nbf=%d pairs=%d prim_pairs=%lld buckets=%d items=%lld\n", nsh, nbf,
@@ -456,6 +498,7 @@ struct Engine {
       mix = cfg->mix_precision != 0;
       mix_thr = cfg->mix_threshold;
       verbose = cfg->verbose;
+      profile = cfg->profile != 0;
     }
     int ndev = 0;
     HIP_CHECK(hipGetDeviceCount(&ndev));
@@ -472,9 +515,16 @@ struct Engine {
     sh_bf0_d.upload(bf0, stream);
     sh_nf_d.upload(nf, stream);
     Pn_d.alloc((size_t)nsh * nsh);
+    Pnd_d.alloc((size_t)nsh * nsh);
     pmax_d.alloc(1);
     bad_d.alloc(1);
     build_pairs();
+    const int ncp = (int)cp_bra.size();
+    cls_ms.assign(2 * (size_t)ncp, 0.0);
+    if (profile) {
+      cls_ev.assign(4 * (size_t)ncp, nullptr);
+      for (auto& e : cls_ev) HIP_CHECK(hipEventCreate(&e));
+    }
   }
 
   // J/K for one density on device buffers
@@ -483,31 +533,34 @@ struct Engine {
     const size_t nn = (size_t)nbf * nbf;
     HIP_CHECK(hipEventRecord(ev[0], s));
     HIP_CHECK(hipMemsetAsync(bad_d.p, 0, sizeof(int), s));
-    launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, bad_d.p, pmax_d.p, s);
+    launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, Pnd_d.p, bad_d.p, pmax_d.p, s);
     HIP_CHECK(hipGetLastError());
     const long long nitems = (long long)items_h.size();
     launch_task_count(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, counts_d.p, s);
+                      pmax_d.p, eri_thr, mix ? mix_thr : -1.0, counts_d.p, s);
     HIP_CHECK(hipGetLastError());
-    launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, nitems + 1, s);
+    launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, 2 * nitems + 1, s);
     HIP_CHECK(hipGetLastError());
     const int ncp = (int)cp_bra.size();
-    HIP_CHECK(hipMemsetAsync(cstat_d.p, 0, sizeof(double) * 2 * ncp, s));
+    HIP_CHECK(hipMemsetAsync(cstat_d.p, 0, sizeof(double) * 4 * ncp, s));
     launch_class_stats(counts_d.p, items_d.p, nitems, item_cp_d.p, buckets_d.p, pairs_d.p, ncp, cstat_d.p,
                        cstat_d.p + ncp, s);
+    launch_class_stats(counts_d.p + nitems, items_d.p, nitems, item_cp_d.p, buckets_d.p, pairs_d.p, ncp,
+                       cstat_d.p + 2 * ncp, cstat_d.p + 3 * ncp, s);
     HIP_CHECK(hipGetLastError());
-    launch_gather_offsets(offsets_d.p, bidx_d.p, ncp + 1, bval_d.p, s);
-    std::vector<long long> bval(ncp + 1);
+    launch_gather_offsets(offsets_d.p, bidx_d.p, 2 * (ncp + 1), bval_d.p, s);
+    std::vector<long long> bval(2 * (ncp + 1));
     int bad = 0;
-    HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * (ncp + 1), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * 2 * (ncp + 1), hipMemcpyDeviceToHost, s));
+    long long total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, offsets_d.p + 2 * nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&bad, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(cstat_h.data(), cstat_d.p, sizeof(double) * 2 * ncp, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(cstat_h.data(), cstat_d.p, sizeof(double) * 4 * ncp, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (bad) throw std::runtime_error("Density matrix contains NaN/Inf values.");
-    const long long total = bval[ncp];
     if (total > 0) tasks_d.alloc((size_t)total);
     launch_task_write(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, counts_d.p, offsets_d.p, tasks_d.p, s);
+                      pmax_d.p, eri_thr, mix ? mix_thr : -1.0, counts_d.p, offsets_d.p, tasks_d.p, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ev[1], s));
     if (dJ) HIP_CHECK(hipMemsetAsync(dJ, 0, nn * sizeof(double), s));
@@ -525,14 +578,21 @@ struct Engine {
       Kw = K_d.p;
       HIP_CHECK(hipMemsetAsync(Kw, 0, nn * sizeof(double), s));
     }
-    long long nq = 0;
-    for (int cp = 0; cp < ncp; ++cp) {
-      const long long n = bval[cp + 1] - bval[cp];
-      if (n <= 0) continue;
-      nq += n;
-      JKLaunch a{tasks_d.p + bval[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jw, Kw, nbf, kscale};
-      launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
-      HIP_CHECK(hipGetLastError());
+    long long nq = 0, nq32 = 0;
+    for (int prec = 0; prec < 2; ++prec) {
+      const long long* b = bval.data() + prec * (ncp + 1);
+      for (int cp = 0; cp < ncp; ++cp) {
+        const long long n = b[cp + 1] - b[cp];
+        if (n <= 0) continue;
+        (prec ? nq32 : nq) += n;
+        JKLaunch a{tasks_d.p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jw, Kw, nbf, kscale,
+                   prec == 1, ppos32_d.p, pk32_d.p, &ra32};
+        hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
+        if (e) HIP_CHECK(hipEventRecord(e[0], s));
+        launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
+        HIP_CHECK(hipGetLastError());
+        if (e) HIP_CHECK(hipEventRecord(e[1], s));
+      }
     }
     HIP_CHECK(hipEventRecord(ev[2], s));
     launch_symmetrize(dJ, dK, nbf, s);
@@ -543,10 +603,22 @@ struct Engine {
     HIP_CHECK(hipEventElapsedTime(&t01, ev[0], ev[1]));
     HIP_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
     HIP_CHECK(hipEventElapsedTime(&t03, ev[0], ev[3]));
-    stats.quartets = nq;
-    stats.quartets_fp32 = 0;
+    for (int prec = 0; prec < 2; ++prec) {
+      const long long* b = bval.data() + prec * (ncp + 1);
+      for (int cp = 0; cp < ncp; ++cp) {
+        double& m = cls_ms[(size_t)prec * ncp + cp];
+        m = 0.0;
+        if (profile && b[cp + 1] > b[cp]) {
+          float t = 0.f;
+          HIP_CHECK(hipEventElapsedTime(&t, cls_ev[4 * (size_t)cp + 2 * prec], cls_ev[4 * (size_t)cp + 2 * prec + 1]));
+          m = t;
+        }
+      }
+    }
+    stats.quartets = nq + nq32;
+    stats.quartets_fp32 = nq32;
     double npq = 0.0;
-    for (int cp = 0; cp < ncp; ++cp) npq += cstat_h[ncp + cp];
+    for (int cp = 0; cp < ncp; ++cp) npq += cstat_h[ncp + cp] + cstat_h[3 * ncp + cp];
     stats.prim_quartets = (long long)npq;
     stats.ms_screen = t01;
     stats.ms_eri = t12;
@@ -649,6 +721,15 @@ int qfe_gradients(qfe_handle* h, const double* P, double* dJ, double* dK, double
   });
 }
 
+int qfe_set_precision(qfe_handle* h, int mix, double threshold) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    if (mix && !(threshold > 0.0)) throw std::invalid_argument("mixed-precision threshold must be positive");
+    h->e.mix = mix != 0;
+    if (mix) h->e.mix_thr = threshold;
+  });
+}
+
 int qfe_get_stats(const qfe_handle* h, qfe_stats* out) {
   return guarded([&] {
     if (!h || !out) throw std::invalid_argument("null argument");
@@ -656,7 +737,8 @@ int qfe_get_stats(const qfe_handle* h, qfe_stats* out) {
   });
 }
 
-int qfe_class_stats(const qfe_handle* h, int cap, int* cls, double* quartets, double* prim_quartets) {
+int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, double* quartets,
+                    double* prim_quartets, double* ms) {
   if (!h) return -1;
   const auto& e = h->e;
   const int ncp = (int)e.cp_bra.size();
@@ -671,8 +753,10 @@ int qfe_class_stats(const qfe_handle* h, int cap, int* cls, double* quartets, do
     cls[4 * cp + 1] = b - la(b) * (la(b) + 1) / 2;
     cls[4 * cp + 2] = la(k);
     cls[4 * cp + 3] = k - la(k) * (la(k) + 1) / 2;
-    quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[cp];
-    prim_quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[ncp + cp];
+    const size_t o = precision ? 2 * (size_t)ncp : 0;
+    quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[o + cp];
+    prim_quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[o + ncp + cp];
+    if (ms) ms[cp] = e.cls_ms.empty() ? 0.0 : e.cls_ms[(precision ? ncp : 0) + cp];
   }
   return ncp;
 }

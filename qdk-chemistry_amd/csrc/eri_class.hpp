@@ -44,11 +44,11 @@ QFE_DI double wave_sum(double v) {
 }
 
 // ------------------------------------------------------------ J/K kernel
-template <class C, int Q0, int Q1>
+template <class C, int Q0, int Q1, typename R>
 __global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, long long ntask,
                                                 const PairHdr* __restrict__ pairs,
-                                                const double4* __restrict__ ppos,
-                                                const double* __restrict__ pk, RysArgs ra,
+                                                const typename Vec4<R>::type* __restrict__ ppos,
+                                                const R* __restrict__ pk, RysArgsT<R> ra,
                                                 const double* __restrict__ P, double* __restrict__ J,
                                                 double* __restrict__ K, int nbf, double kscale) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,8 +57,8 @@ __global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, 
   const PairHdr bra = pairs[tk.x];
   const PairHdr ket = pairs[tk.y];
   constexpr int NQ = Q1 - Q0;
-  double acc[NQ][C::NKETC];
-  contract_chunk<C, Q0, Q1>(bra, ket, ppos, pk, ra, acc);
+  R acc[NQ][C::NKETC];
+  contract_chunk<C, Q0, Q1, R>(bra, ket, ppos, pk, ra, acc);
 
   const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
                      (tk.x == tk.y ? 1.0 : 2.0);
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, 
     constexpr int ia = ma - MA0;
     const size_t ri = (size_t)(i0 + ma) * n, rj = (size_t)(j0 + mb) * n;
     double v[C::NKETS];
-    ket_to_sph<C>(acc[qq], v);
+    ket_to_sph<C, R>(acc[qq], v);
     const double Pij = P[ri + j0 + mb];
     double jij = 0.0;
     sfor<C::NSC>([&](auto c) {
@@ -150,11 +150,11 @@ __global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ plist, 
   const int p = plist[t];
   const PairHdr pr = pairs[p];
   double acc[Q1 - Q0][C::NKETC];
-  contract_chunk<C, Q0, Q1>(pr, pr, ppos, pk, ra, acc);
+  contract_chunk<C, Q0, Q1, double>(pr, pr, ppos, pk, ra, acc);
   double s = 0.0;
   sfor<Q1 - Q0>([&](auto qq) {
     double v[C::NKETS];
-    ket_to_sph<C>(acc[qq], v);
+    ket_to_sph<C, double>(acc[qq], v);
     sfor<C::NKETS>([&](auto x) { s = fma(v[x], v[x], s); });
   });
   unsafeAtomicAdd(&q2[p], s);
@@ -174,8 +174,12 @@ struct ClassOps {
     hsfor<NCH>([&](auto c) {
       constexpr int q0 = c * CH;
       constexpr int q1 = (q0 + CH < C::NBRA) ? q0 + CH : C::NBRA;
-      hipLaunchKernelGGL((k_eri_jk<C, q0, q1>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
-                         a.ntask, a.pairs, a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale);
+      if (a.fp32)
+        hipLaunchKernelGGL((k_eri_jk<C, q0, q1, float>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
+                           a.ntask, a.pairs, a.ppos32, a.pk32, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale);
+      else
+        hipLaunchKernelGGL((k_eri_jk<C, q0, q1, double>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
+                           a.ntask, a.pairs, a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale);
     });
   }
   static void schwarz(const SchwarzLaunch& a, hipStream_t s) {

@@ -73,12 +73,26 @@ struct PairHdr {        // one significant shell pair (80 B)
 };
 
 // primitive pair: ppos = (p, Px, Py, Pz); pk = c_a c_b exp(-ab/p |AB|^2) / p
-struct RysArgs {
-  const double* coef[9];   // per root count n: [nint][2n][RYS_DEG+1]
-  double tmax[9];
-  double hx2[9][8];        // large-T limit: u_i = hx2/T
-  double hw[9][8];         //               w_i = hw/sqrt(T)
-  const double* boys;      // Boys Taylor table [BOYS_NG][BOYS_W]: F_m(k * BOYS_DT)
+template <typename R>
+struct RysArgsT {
+  const R* coef[9];   // per root count n: [nint][2n][RYS_DEG+1]
+  R tmax[9];
+  R hx2[9][8];        // large-T limit: u_i = hx2/T
+  R hw[9][8];         //               w_i = hw/sqrt(T)
+  const R* boys;      // Boys Taylor table [BOYS_NG][BOYS_W]: F_m(k * BOYS_DT)
+};
+using RysArgs = RysArgsT<double>;
+using RysArgs32 = RysArgsT<float>;
+
+template <typename R>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
 };
 constexpr int RYS_DEG = 11;
 constexpr double BOYS_DT = 0.05;
@@ -99,6 +113,17 @@ QFE_DI double frsq(double x) {
   r = r * fma(-hx * r, r, 1.5);
   return r * fma(-hx * r, r, 1.5);
 }
+// single precision: one Newton step on the hardware estimate
+QFE_DI float frcp(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+QFE_DI float frsq(float x) {
+  const float r = __builtin_amdgcn_rsqf(x);
+  return r * fmaf(-0.5f * x * r, r, 1.5f);
+}
+QFE_DI double fexp(double x) { return exp(x); }
+QFE_DI float fexp(float x) { return expf(x); }
 
 // F_0..F_M at T.  Below BOYS_TSW: 7-term Taylor expansion about the nearest
 // grid point (|dT| <= 0.025, truncation < 2e-16 relative); for M <= 2 every
@@ -106,59 +131,59 @@ QFE_DI double frsq(double x) {
 // downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) fills the rest.  Above:
 // F_0 from the asymptotic form and the upward recursion (stable for T > M;
 // e^-T < 4e-18 is dropped).
-template <int M>
-QFE_DI void boys_fm(double T, const double* __restrict__ tab, double (&F)[M + 1]) {
-  if (T < BOYS_TSW) {
-    const int g = (int)(T * (1.0 / BOYS_DT) + 0.5);
-    const double d = (double)g * BOYS_DT - T;
-    const double* __restrict__ row = tab + (size_t)g * BOYS_W;
+template <int M, typename R>
+QFE_DI void boys_fm(R T, const R* __restrict__ tab, R (&F)[M + 1]) {
+  if (T < R(BOYS_TSW)) {
+    const int g = (int)(T * R(1.0 / BOYS_DT) + R(0.5));
+    const R d = (R)g * R(BOYS_DT) - T;
+    const R* __restrict__ row = tab + (size_t)g * BOYS_W;
     if constexpr (M <= 2) {
       sfor<M + 1>([&](auto m) {
-        double s = row[m + 6];
+        R s = row[m + 6];
         sfor<6>([&](auto kk) {
           constexpr int k = 5 - kk;
-          s = fma(s, d * (1.0 / (double)(k + 1)), row[m + k]);
+          s = fma(s, d * R(1.0 / (double)(k + 1)), row[m + k]);
         });
         F[m] = s;
       });
     } else {
-      const double e = exp(-T);
-      double s = row[M + 6];
+      const R e = fexp(-T);
+      R s = row[M + 6];
       sfor<6>([&](auto kk) {
         constexpr int k = 5 - kk;
-        s = fma(s, d * (1.0 / (double)(k + 1)), row[M + k]);
+        s = fma(s, d * R(1.0 / (double)(k + 1)), row[M + k]);
       });
       F[M] = s;
       sfor<M>([&](auto mm) {
         constexpr int m = M - 1 - mm;
-        F[m] = (2.0 * T * F[m + 1] + e) * (1.0 / (double)(2 * m + 1));
+        F[m] = (R(2) * T * F[m + 1] + e) * R(1.0 / (double)(2 * m + 1));
       });
     }
   } else {
-    const double it = frcp(T);
-    F[0] = 0.88622692545275801365 * frsq(T);  // sqrt(pi)/2 / sqrt(T)
-    sfor<M>([&](auto m) { F[m + 1] = (double)(2 * m + 1) * F[m] * (0.5 * it); });
+    const R it = frcp(T);
+    F[0] = R(0.88622692545275801365) * frsq(T);  // sqrt(pi)/2 / sqrt(T)
+    sfor<M>([&](auto m) { F[m + 1] = R(2 * m + 1) * F[m] * (R(0.5) * it); });
   }
 }
 
-template <int N>
-QFE_DI void rys_roots(double T, const RysArgs& ra, double (&u)[N], double (&w)[N]) {
+template <int N, typename R>
+QFE_DI void rys_roots(R T, const RysArgsT<R>& ra, R (&u)[N], R (&w)[N]) {
   if constexpr (N <= 2) {
     // Gauss rule of the Boys measure from its moments F_0..F_{2N-1}
-    double F[2 * N];
+    R F[2 * N];
     boys_fm<2 * N - 1>(T, ra.boys, F);
     if constexpr (N == 1) {
       w[0] = F[0];
       u[0] = F[1] * frcp(F[0]);
     } else {
       // monic pi_2(x) = x^2 + a x + b orthogonal to 1 and x
-      const double idet = frcp(F[0] * F[2] - F[1] * F[1]);
-      const double a = (F[1] * F[2] - F[0] * F[3]) * idet;
-      const double b = (F[1] * F[3] - F[2] * F[2]) * idet;
-      const double disc = a * a - 4.0 * b;
-      const double q = 0.5 * (disc * frsq(disc) - a);  // larger root (a < 0)
-      const double x1 = q, x0 = b * frcp(q);
-      const double w0 = (F[1] - x1 * F[0]) * frcp(x0 - x1);
+      const R idet = frcp(F[0] * F[2] - F[1] * F[1]);
+      const R a = (F[1] * F[2] - F[0] * F[3]) * idet;
+      const R b = (F[1] * F[3] - F[2] * F[2]) * idet;
+      const R disc = a * a - R(4) * b;
+      const R q = R(0.5) * (disc * frsq(disc) - a);  // larger root (a < 0)
+      const R x1 = q, x0 = b * frcp(q);
+      const R w0 = (F[1] - x1 * F[0]) * frcp(x0 - x1);
       u[0] = x0;
       u[1] = x1;
       w[0] = w0;
@@ -166,20 +191,20 @@ QFE_DI void rys_roots(double T, const RysArgs& ra, double (&u)[N], double (&w)[N
     }
   } else {
     if (T >= ra.tmax[N]) {
-      const double it = frcp(T);
-      const double is = frsq(T);
+      const R it = frcp(T);
+      const R is = frsq(T);
       sfor<N>([&](auto i) {
         u[i] = ra.hx2[N][i] * it;
         w[i] = ra.hw[N][i] * is;
       });
     } else {
       const int k = (int)T;
-      const double x = T - ((double)k + 0.5);
-      const double* __restrict__ c = ra.coef[N] + (size_t)k * (2 * N) * (RYS_DEG + 1);
+      const R x = T - ((R)k + R(0.5));
+      const R* __restrict__ c = ra.coef[N] + (size_t)k * (2 * N) * (RYS_DEG + 1);
       sfor<N>([&](auto i) {
-        const double* cu = c + i * (RYS_DEG + 1);
-        const double* cw = c + (N + i) * (RYS_DEG + 1);
-        double vu = cu[RYS_DEG], vw = cw[RYS_DEG];
+        const R* cu = c + i * (RYS_DEG + 1);
+        const R* cw = c + (N + i) * (RYS_DEG + 1);
+        R vu = cu[RYS_DEG], vw = cw[RYS_DEG];
 #pragma unroll
         for (int d = RYS_DEG - 1; d >= 0; --d) {
           vu = fma(vu, x, cu[d]);
@@ -209,45 +234,42 @@ struct Cls {
 };
 
 // 1-D integrals of one direction for one root: I[a][b][c][d]
-template <class C>
+template <class C, typename R>
 struct OneD {
-  double v[C::LA + 1][C::LB + 1][C::LC + 1][C::LD + 1];
+  R v[C::LA + 1][C::LB + 1][C::LC + 1][C::LD + 1];
 };
 
 // 2-D recursion + 1-D horizontal transfer for one direction
-template <class C>
-QFE_DI void rys_1d(double s, double C00, double C00p, double B00, double B10, double B01,
-                   double AB, double CD, OneD<C>& out) {
+template <class C, typename R>
+QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, R>& out) {
   constexpr int EA = C::EA, FC = C::FC;
-  double g[EA + 1][FC + 1];
+  R g[EA + 1][FC + 1];
   g[0][0] = s;
   if constexpr (EA >= 1) g[1][0] = C00 * s;
   sfor<(EA >= 2 ? EA - 1 : 0)>([&](auto ii) {
     constexpr int i = ii + 1;
-    g[i + 1][0] = C00 * g[i][0] + (double)i * B10 * g[i - 1][0];
+    g[i + 1][0] = C00 * g[i][0] + R(i) * B10 * g[i - 1][0];
   });
   if constexpr (FC >= 1) {
     g[0][1] = C00p * s;
     sfor<EA>([&](auto ii) {
       constexpr int i = ii + 1;
-      g[i][1] = C00p * g[i][0] + (double)i * B00 * g[i - 1][0];
+      g[i][1] = C00p * g[i][0] + R(i) * B00 * g[i - 1][0];
     });
     sfor<(FC >= 2 ? FC - 1 : 0)>([&](auto kk) {
       constexpr int k = kk + 1;
-      g[0][k + 1] = C00p * g[0][k] + (double)k * B01 * g[0][k - 1];
+      g[0][k + 1] = C00p * g[0][k] + R(k) * B01 * g[0][k - 1];
       sfor<EA>([&](auto ii) {
         constexpr int i = ii + 1;
-        g[i][k + 1] = C00p * g[i][k] + (double)k * B01 * g[i][k - 1] + (double)i * B00 * g[i - 1][k];
+        g[i][k + 1] = C00p * g[i][k] + R(k) * B01 * g[i][k - 1] + R(i) * B00 * g[i - 1][k];
       });
     });
   }
   // ket transfer: h[i][c][d] for i <= EA, from g[i][c + d]
-  double h[EA + 1][C::LC + 1][C::LD + 1];
+  R h[EA + 1][C::LC + 1][C::LD + 1];
   sfor<EA + 1>([&](auto i) {
-    // level d = 0
-    sfor<C::LC + 1>([&](auto c) { h[i][c][0] = g[i][c]; });
-    // d > 0: (c, d) = (c+1, d-1) + CD (c, d-1); needs c + d <= FC
-    double t[FC + 1][C::LD + 1];
+    // (c, d) = (c+1, d-1) + CD (c, d-1); needs c + d <= FC
+    R t[FC + 1][C::LD + 1];
     sfor<FC + 1>([&](auto k) { t[k][0] = g[i][k]; });
     sfor<C::LD>([&](auto dd) {
       constexpr int d = dd + 1;
@@ -258,7 +280,7 @@ QFE_DI void rys_1d(double s, double C00, double C00p, double B00, double B10, do
   // bra transfer: I[a][b][c][d] from h[a + b][c][d]
   sfor<C::LC + 1>([&](auto c) {
     sfor<C::LD + 1>([&](auto d) {
-      double t[EA + 1][C::LB + 1];
+      R t[EA + 1][C::LB + 1];
       sfor<EA + 1>([&](auto i) { t[i][0] = h[i][c][d]; });
       sfor<C::LB>([&](auto bb) {
         constexpr int b = bb + 1;
@@ -271,46 +293,51 @@ QFE_DI void rys_1d(double s, double C00, double C00p, double B00, double B10, do
 
 // Contract all primitive quartets of (bra|ket) into acc[q][kc] for the bra
 // function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and all ket
-// cartesian pairs kc (row-major over (c, d)).
-template <class C, int Q0, int Q1>
-QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket, const double4* __restrict__ ppos,
-                           const double* __restrict__ pk, const RysArgs& ra,
-                           double (&acc)[Q1 - Q0][C::NKETC]) {
-  constexpr double TWO_PI_52 = 34.98683665524972;  // 2 pi^(5/2)
-  sfor<Q1 - Q0>([&](auto q) { sfor<C::NKETC>([&](auto k) { acc[q][k] = 0.0; }); });
-  const double ABx = bra.A[0] - bra.B[0], ABy = bra.A[1] - bra.B[1], ABz = bra.A[2] - bra.B[2];
-  const double CDx = ket.A[0] - ket.B[0], CDy = ket.A[1] - ket.B[1], CDz = ket.A[2] - ket.B[2];
-  for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
-    const double4 P = ppos[ib];
-    const double Kb = pk[ib];
-    const double p = P.x;
-    const double PAx = P.y - bra.A[0], PAy = P.z - bra.A[1], PAz = P.w - bra.A[2];
-    for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
-      const double4 Q = ppos[ik];
-      const double Kk = pk[ik];
-      const double q = Q.x;
-      const double QCx = Q.y - ket.A[0], QCy = Q.z - ket.A[1], QCz = Q.w - ket.A[2];
-      const double PQx = P.y - Q.y, PQy = P.z - Q.z, PQz = P.w - Q.w;
-      const double rs = frsq(p + q);
-      const double rpq = rs * rs;
-      const double rho = p * q * rpq;
-      const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
-      const double pref = TWO_PI_52 * Kb * Kk * rs;
-      double u[C::NR], w[C::NR];
+// cartesian pairs kc (row-major over (c, d)), in precision R.
+template <class C, int Q0, int Q1, typename R>
+QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
+                           const typename Vec4<R>::type* __restrict__ ppos, const R* __restrict__ pk,
+                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][C::NKETC]) {
+  const R TWO_PI_52 = R(34.98683665524972);  // 2 pi^(5/2)
+  sfor<Q1 - Q0>([&](auto q) { sfor<C::NKETC>([&](auto k) { acc[q][k] = R(0); }); });
+  const R ABx = R(bra.A[0] - bra.B[0]), ABy = R(bra.A[1] - bra.B[1]), ABz = R(bra.A[2] - bra.B[2]);
+  const R CDx = R(ket.A[0] - ket.B[0]), CDy = R(ket.A[1] - ket.B[1]), CDz = R(ket.A[2] - ket.B[2]);
+  const R Ax = R(bra.A[0]), Ay = R(bra.A[1]), Az = R(bra.A[2]);
+  const R Cx = R(ket.A[0]), Cy = R(ket.A[1]), Cz = R(ket.A[2]);
+  // ket primitives outer (per-lane data), bra primitives inner: the bra pair
+  // is usually shared by the whole wave, so the inner loads are broadcasts
+  for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
+    const auto Q = ppos[ik];
+    const R Kk = pk[ik];
+    const R q = Q.x;
+    const R QCx = Q.y - Cx, QCy = Q.z - Cy, QCz = Q.w - Cz;
+    const R h2q = R(0.5) * frcp(q);
+    for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
+      const auto P = ppos[ib];
+      const R Kb = pk[ib];
+      const R p = P.x;
+      const R PAx = P.y - Ax, PAy = P.z - Ay, PAz = P.w - Az;
+      const R PQx = P.y - Q.y, PQy = P.z - Q.z, PQz = P.w - Q.w;
+      const R rs = frsq(p + q);
+      const R rpq = rs * rs;
+      const R rho = p * q * rpq;
+      const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const R pref = TWO_PI_52 * Kb * Kk * rs;
+      R u[C::NR], w[C::NR];
       rys_roots<C::NR>(T, ra, u, w);
-      const double qr = q * rpq, pr = p * rpq;
-      const double h2p = 0.5 * frcp(p), h2q = 0.5 * frcp(q);
+      const R qr = q * rpq, pr = p * rpq;
+      const R h2p = R(0.5) * frcp(p);
 #pragma unroll 1
       for (int r = 0; r < C::NR; ++r) {
-        const double ur = u[r];
-        const double B00 = 0.5 * rpq * ur;
-        const double B10 = h2p - h2p * qr * ur;
-        const double B01 = h2q - h2q * pr * ur;
-        const double xa = qr * ur, xc = pr * ur;
-        OneD<C> Ix, Iy, Iz;
-        rys_1d<C>(1.0, PAx - xa * PQx, QCx + xc * PQx, B00, B10, B01, ABx, CDx, Ix);
-        rys_1d<C>(1.0, PAy - xa * PQy, QCy + xc * PQy, B00, B10, B01, ABy, CDy, Iy);
-        rys_1d<C>(w[r] * pref, PAz - xa * PQz, QCz + xc * PQz, B00, B10, B01, ABz, CDz, Iz);
+        const R ur = u[r];
+        const R B00 = R(0.5) * rpq * ur;
+        const R B10 = h2p - h2p * qr * ur;
+        const R B01 = h2q - h2q * pr * ur;
+        const R xa = qr * ur, xc = pr * ur;
+        OneD<C, R> Ix, Iy, Iz;
+        rys_1d<C, R>(R(1), PAx - xa * PQx, QCx + xc * PQx, B00, B10, B01, ABx, CDx, Ix);
+        rys_1d<C, R>(R(1), PAy - xa * PQy, QCy + xc * PQy, B00, B10, B01, ABy, CDy, Iy);
+        rys_1d<C, R>(w[r] * pref, PAz - xa * PQz, QCz + xc * PQz, B00, B10, B01, ABz, CDz, Iz);
         // assemble: bra in output functions, ket in cartesians
         sfor<Q1 - Q0>([&](auto qq) {
           constexpr int qi = Q0 + qq;
@@ -319,7 +346,7 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket, const double4
             constexpr int cx = cart_x(C::LC, cc), cy = cart_y(C::LC, cc), cz = cart_z(C::LC, cc);
             sfor<C::NCD>([&](auto dc) {
               constexpr int dx = cart_x(C::LD, dc), dy = cart_y(C::LD, dc), dz = cart_z(C::LD, dc);
-              double s = 0.0;
+              R s = R(0);
               sfor<C::NCA>([&](auto ac) {
                 constexpr double ca = sph_coef(C::LA, C::PURE, ma, ac);
                 if constexpr (ca != 0.0) {
@@ -328,11 +355,11 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket, const double4
                     constexpr double cb = sph_coef(C::LB, C::PURE, mb, bc);
                     if constexpr (cb != 0.0) {
                       constexpr int bx = cart_x(C::LB, bc), by = cart_y(C::LB, bc), bz = cart_z(C::LB, bc);
-                      const double v = Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz];
+                      const R v = Ix.v[ax][bx][cx][dx] * Iy.v[ay][by][cy][dy] * Iz.v[az][bz][cz][dz];
                       if constexpr (ca * cb == 1.0)
                         s += v;
                       else
-                        s = fma(ca * cb, v, s);
+                        s = fma(R(ca * cb), v, s);
                     }
                   });
                 }
@@ -347,8 +374,8 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket, const double4
 }
 
 // ket cartesian -> output functions for one bra function pair
-template <class C>
-QFE_DI void ket_to_sph(const double (&in)[C::NKETC], double (&out)[C::NKETS]) {
+template <class C, typename R>
+QFE_DI void ket_to_sph(const R (&in)[C::NKETC], double (&out)[C::NKETS]) {
   sfor<C::NSC>([&](auto mc) {
     sfor<C::NSD>([&](auto md) {
       double s = 0.0;
@@ -357,7 +384,7 @@ QFE_DI void ket_to_sph(const double (&in)[C::NKETC], double (&out)[C::NKETS]) {
         if constexpr (c1 != 0.0) {
           sfor<C::NCD>([&](auto dc) {
             constexpr double c2 = sph_coef(C::LD, C::PURE, md, dc);
-            if constexpr (c2 != 0.0) s = fma(c1 * c2, in[cc * C::NCD + dc], s);
+            if constexpr (c2 != 0.0) s = fma(c1 * c2, (double)in[cc * C::NCD + dc], s);
           });
         }
       });

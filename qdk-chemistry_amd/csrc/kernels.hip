@@ -15,9 +15,11 @@
 namespace qfe {
 
 // ------------------------------------------------------------ density norms
+// Pn is kept in float rounded towards +inf, so the screening test can only
+// keep more quartets than the double-precision test of the reference
 __global__ void k_shell_norm(const double* __restrict__ P, int nbf, const int* __restrict__ sh_bf0,
-                             const int* __restrict__ sh_nf, int nsh, double* __restrict__ Pn,
-                             int* __restrict__ bad) {
+                             const int* __restrict__ sh_nf, int nsh, float* __restrict__ Pn,
+                             double* __restrict__ Pnd, int* __restrict__ bad) {
   const int s2 = blockIdx.x * blockDim.x + threadIdx.x;
   const int s1 = blockIdx.y;
   if (s2 >= nsh) return;
@@ -29,7 +31,8 @@ __global__ void k_shell_norm(const double* __restrict__ P, int nbf, const int* _
       if (!(v <= 1.0e300)) *bad = 1;  // NaN or Inf
       m = fmax(m, v);
     }
-  Pn[(size_t)s1 * nsh + s2] = m;
+  Pn[(size_t)s1 * nsh + s2] = __double2float_ru(m);
+  Pnd[(size_t)s1 * nsh + s2] = m;
 }
 
 __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __restrict__ out) {
@@ -52,32 +55,34 @@ __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __r
 // ------------------------------------------------------------ task list
 // work item = (bra pair p, ket bucket b); one wavefront scans bucket b.
 __device__ __forceinline__ bool quartet_passes(int2 P1, double Q1, int2 P2, double Q2,
-                                               const double* __restrict__ Pn, int nsh, double thr) {
+                                               const float* __restrict__ Pn, int nsh, double thr) {
   const size_t a = P1.x, b = P1.y, c = P2.x, d = P2.y;
-  double m = Pn[a * nsh + b];
-  m = fmax(m, Pn[c * nsh + d]);
-  m = fmax(m, Pn[a * nsh + c]);
-  m = fmax(m, Pn[a * nsh + d]);
-  m = fmax(m, Pn[b * nsh + c]);
-  m = fmax(m, Pn[b * nsh + d]);
-  return !(m * Q1 * Q2 < thr);
+  float m = Pn[a * nsh + b];
+  m = fmaxf(m, Pn[c * nsh + d]);
+  m = fmaxf(m, Pn[a * nsh + c]);
+  m = fmaxf(m, Pn[a * nsh + d]);
+  m = fmaxf(m, Pn[b * nsh + c]);
+  m = fmaxf(m, Pn[b * nsh + d]);
+  return !((double)m * Q1 * Q2 < thr);
 }
 
+// counts/offsets are [2][nitems]: the FP64 part, then the FP32 part (quartets
+// whose Schwarz bound Q_ab Q_cd is below mix_thr when mixed precision is on).
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ items, long long nitems,
                                                    const int2* __restrict__ buckets,
                                                    const int* __restrict__ pair_bucket,
                                                    const int2* __restrict__ psh,
                                                    const double* __restrict__ Q,
-                                                   const double* __restrict__ Pn, int nsh,
+                                                   const float* __restrict__ Pn, int nsh,
                                                    const double* __restrict__ pmax_ptr, double thr,
-                                                   int* __restrict__ counts,
+                                                   double mix_thr, int* __restrict__ counts,
                                                    const long long* __restrict__ offsets,
                                                    int2* __restrict__ tasks) {
   const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (item >= nitems) return;
-  if (WRITE && counts[item] == 0) return;
+  if (WRITE && counts[item] == 0 && counts[nitems + item] == 0) return;
   const int2 it = items[item];
   const int p = it.x;
   const int2 bk = buckets[it.y];
@@ -86,25 +91,35 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
   const double Qp = Q[p];
   const int2 Pp = psh[p];
   const double cut = thr / (*pmax_ptr);
-  long long base = WRITE ? offsets[item] : 0;
-  int count = 0;
+  const long long base64 = WRITE ? offsets[item] : 0;
+  const long long base32 = WRITE ? offsets[nitems + item] : 0;
+  int c64 = 0, c32 = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
   for (int q0 = bk.x; q0 < end; q0 += 64) {
     // sorted by Q descending: if the first ket fails the Schwarz bound, all later do
     if (Qp * Q[q0] < cut) break;
     const int q = q0 + lane;
-    bool ok = false;
+    bool ok = false, lo = false;
     if (q < end) {
       const double Qq = Q[q];
       ok = quartet_passes(Pp, Qp, psh[q], Qq, Pn, nsh, thr);
+      lo = Qp * Qq < mix_thr;
     }
-    const unsigned long long m = __ballot(ok);
+    const unsigned long long m64 = __ballot(ok && !lo);
+    const unsigned long long m32 = __ballot(ok && lo);
     if (WRITE && ok) {
-      const int pos = __popcll(m & ((1ull << lane) - 1ull));
-      tasks[base + count + pos] = make_int2(p, q);
+      if (lo)
+        tasks[base32 + c32 + __popcll(m32 & below)] = make_int2(p, q);
+      else
+        tasks[base64 + c64 + __popcll(m64 & below)] = make_int2(p, q);
     }
-    count += __popcll(m);
+    c64 += __popcll(m64);
+    c32 += __popcll(m32);
   }
-  if (!WRITE && lane == 0) counts[item] = count;
+  if (!WRITE && lane == 0) {
+    counts[item] = c64;
+    counts[nitems + item] = c32;
+  }
 }
 
 // per class pair: surviving shell quartets and their primitive quartets
@@ -165,36 +180,36 @@ __global__ void k_symmetrize(double* __restrict__ J, double* __restrict__ K, int
 
 // ------------------------------------------------------------ host wrappers
 void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
-                       double* Pn, int* bad, double* pmax, hipStream_t s) {
+                       float* Pn, double* Pnd, int* bad, double* pmax, hipStream_t s) {
   dim3 block(64), grid((nsh + 63) / 64, nsh);
-  hipLaunchKernelGGL(k_shell_norm, grid, block, 0, s, P, nbf, sh_bf0, sh_nf, nsh, Pn, bad);
+  hipLaunchKernelGGL(k_shell_norm, grid, block, 0, s, P, nbf, sh_bf0, sh_nf, nsh, Pn, Pnd, bad);
   (void)hipMemsetAsync(pmax, 0, sizeof(double), s);
   size_t n = (size_t)nsh * nsh;
   int g = (int)((n + 255) / 256);
   if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_max_reduce, dim3(g), dim3(256), 0, s, Pn, n, pmax);
+  hipLaunchKernelGGL(k_max_reduce, dim3(g), dim3(256), 0, s, Pnd, n, pmax);
 }
 
 void launch_task_count(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
-                       const int2* pairs, const double* Q, const double* Pn, int nsh,
-                       const double* pmax, double thr, int* counts, hipStream_t s) {
+                       const int2* pairs, const double* Q, const float* Pn, int nsh,
+                       const double* pmax, double thr, double mix_thr, int* counts, hipStream_t s) {
   if (nitems <= 0) return;
   const long long threads = nitems * 64;
   const long long grid = (threads + 255) / 256;
   hipLaunchKernelGGL((k_task_scan<false>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
-                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, counts, (const long long*)nullptr,
+                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, mix_thr, counts, (const long long*)nullptr,
                      (int2*)nullptr);
 }
 
 void launch_task_write(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
-                       const int2* pairs, const double* Q, const double* Pn, int nsh,
-                       const double* pmax, double thr, const int* counts, const long long* offsets,
-                       int2* tasks, hipStream_t s) {
+                       const int2* pairs, const double* Q, const float* Pn, int nsh,
+                       const double* pmax, double thr, double mix_thr, const int* counts,
+                       const long long* offsets, int2* tasks, hipStream_t s) {
   if (nitems <= 0) return;
   const long long threads = nitems * 64;
   const long long grid = (threads + 255) / 256;
   hipLaunchKernelGGL((k_task_scan<true>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
-                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, (int*)counts, offsets, tasks);
+                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, mix_thr, (int*)counts, offsets, tasks);
 }
 
 size_t scan_temp_bytes(long long n) {

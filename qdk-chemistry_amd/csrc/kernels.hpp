@@ -10,7 +10,10 @@
 namespace qfe {
 
 struct PairHdr;
-struct RysArgs;
+template <typename R>
+struct RysArgsT;
+using RysArgs = RysArgsT<double>;
+using RysArgs32 = RysArgsT<float>;
 
 struct JKLaunch {
   const int2* tasks;
@@ -24,6 +27,10 @@ struct JKLaunch {
   double* K;
   int nbf;
   double kscale;
+  bool fp32;          // evaluate in single precision (mixed-precision tasks)
+  const float4* ppos32;
+  const float* pk32;
+  const RysArgs32* ra32;
 };
 
 struct SchwarzLaunch {
@@ -39,15 +46,15 @@ struct SchwarzLaunch {
 void launch_jk(int bra_cls, int ket_cls, bool pure, const JKLaunch& a, hipStream_t s);
 void launch_schwarz(int pair_cls, bool pure, const SchwarzLaunch& a, hipStream_t s);
 void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
-                       double* Pn, int* bad, double* pmax, hipStream_t s);
+                       float* Pn, double* Pnd, int* bad, double* pmax, hipStream_t s);
 void launch_task_count(const int2* items, long long nitems, const int2* buckets,
                        const int* pair_bucket, const int2* pair_shells, const double* Q,
-                       const double* Pn, int nsh, const double* pmax, double thr, int* counts,
-                       hipStream_t s);
+                       const float* Pn, int nsh, const double* pmax, double thr, double mix_thr,
+                       int* counts, hipStream_t s);
 void launch_task_write(const int2* items, long long nitems, const int2* buckets,
                        const int* pair_bucket, const int2* pair_shells, const double* Q,
-                       const double* Pn, int nsh, const double* pmax, double thr, const int* counts,
-                       const long long* offsets, int2* tasks, hipStream_t s);
+                       const float* Pn, int nsh, const double* pmax, double thr, double mix_thr,
+                       const int* counts, const long long* offsets, int2* tasks, hipStream_t s);
 size_t scan_temp_bytes(long long n);
 void launch_exclusive_scan(void* temp, size_t temp_bytes, const int* in, long long* out,
                            long long n, hipStream_t s);

@@ -28,7 +28,7 @@ class QfeBasis(ctypes.Structure):
 class QfeConfig(ctypes.Structure):
     _fields_ = [("eri_threshold", ctypes.c_double), ("shell_pair_threshold", ctypes.c_double),
                 ("device_id", ctypes.c_int), ("mix_precision", ctypes.c_int),
-                ("mix_threshold", ctypes.c_double), ("verbose", ctypes.c_int)]
+                ("mix_threshold", ctypes.c_double), ("verbose", ctypes.c_int), ("profile", ctypes.c_int)]
 
 
 class QfeStats(ctypes.Structure):
@@ -40,7 +40,7 @@ class QfeStats(ctypes.Structure):
 
 
 EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients", "qfe_one_body",
-           "qfe_one_body_grad", "qfe_get_stats", "qfe_nbf", "qfe_destroy", "qfe_last_error"]
+           "qfe_one_body_grad", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_nbf", "qfe_destroy", "qfe_last_error"]
 
 _lib = None
 
@@ -69,6 +69,8 @@ def lib():
         L.qfe_one_body.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp]
         L.qfe_one_body_grad.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, _dp]
         L.qfe_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(QfeStats)]
+        L.qfe_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.qfe_class_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp]
         L.qfe_nbf.argtypes = [ctypes.c_void_p]
         L.qfe_destroy.argtypes = [ctypes.c_void_p]
         L.qfe_destroy.restype = None

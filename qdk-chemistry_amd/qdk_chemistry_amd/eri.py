@@ -40,6 +40,7 @@ class ERIConfig:
     mix_precision_threshold: float = 1e-6
     device_id: int = 0
     verbose: int = 0
+    profile: bool = False
 
 
 class RysERI:
@@ -53,10 +54,11 @@ class RysERI:
         self.nbf = basis.nbf
         self._b = _lib.BasisArrays(basis, structure)
         c = _lib.QfeConfig(cfg.eri_threshold, cfg.shell_pair_threshold, cfg.device_id,
-                           int(cfg.is_mix_precision), cfg.mix_precision_threshold, cfg.verbose)
+                           int(cfg.is_mix_precision), cfg.mix_precision_threshold, cfg.verbose, int(cfg.profile))
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().qfe_create(ctypes.byref(self._b.c), ctypes.byref(c), ctypes.byref(h)))
         self.h = h
+        self.mixed = bool(cfg.is_mix_precision)
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -94,6 +96,23 @@ class RysERI:
         dK = np.zeros(3 * na)
         _lib.check(_lib.lib().qfe_gradients(self.h, _lib.ptr(P), _lib.ptr(dJ), _lib.ptr(dK), alpha, beta, omega))
         return dJ, dK
+
+    def set_precision(self, mix: bool, threshold: float | None = None):
+        """Mixed (FP32 for Q_ab Q_cd < threshold) or all-FP64 builds from now on."""
+        thr = self.cfg.mix_precision_threshold if threshold is None else threshold
+        _lib.check(_lib.lib().qfe_set_precision(self.h, int(mix), thr))
+        self.mixed = bool(mix)
+
+    def class_stats(self, precision=0):
+        """[((la, lb, lc, ld), quartets, prim_quartets, ms)] of the last build,
+        for the FP64 (0) or FP32 (1) tasks."""
+        cap = 256
+        cls = np.zeros(4 * cap, dtype=np.int32)
+        q, pq, ms = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        n = _lib.lib().qfe_class_stats(self.h, precision, cap, _lib.ptr(cls, ctypes.c_int), _lib.ptr(q),
+                                       _lib.ptr(pq), _lib.ptr(ms))
+        return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), float(q[i]), float(pq[i]), float(ms[i]))
+                for i in range(min(n, cap))]
 
     def stats(self) -> dict:
         s = _lib.QfeStats()

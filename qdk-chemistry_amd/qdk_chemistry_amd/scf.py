@@ -34,7 +34,11 @@ class SCFConfig:
     fock_reset_steps: int = 1073741824
     diis_size: int = 10
     lindep_threshold: float = -1.0
-    fp64_final_iterations: int = 3      # mixed precision: last builds in FP64
+    # mixed precision: once the orbital gradient drops below this value the
+    # driver resets the Fock matrix with a full FP64 build and finishes the
+    # SCF with FP64 incremental builds (DIIS, orthogonalisation and
+    # diagonalisation are FP64 throughout)
+    fp64_switch_gradient: float = 1e-5
     eri: ERIConfig = field(default_factory=ERIConfig)
 
 
@@ -160,12 +164,24 @@ class RHF:
         E_last = 0.0
         diis_F, diis_E = [], []
         converged = False
+        mixed = bool(getattr(self.eri, "mixed", False))
+        self.mixed_builds = 0
+        reset_now = False
+        og = np.inf
         for step in range(cfg.max_iterations):
-            if step < cfg.incremental_fock_start_step or step % cfg.fock_reset_steps == 0:
+            if mixed and og < cfg.fp64_switch_gradient:
+                # final iterations in FP64, starting from a full FP64 rebuild
+                self.eri.set_precision(False)
+                mixed = False
+                reset_now = True
+            if step < cfg.incremental_fock_start_step or step % cfg.fock_reset_steps == 0 or reset_now:
                 P_diff = P
                 F = self.H.copy()
+                reset_now = False
             else:
                 P_diff = P - P_last
+            if mixed:
+                self.mixed_builds += 1
             P_last = P.copy()
             J, K = self._jk(P_diff)
             F = F + J - 0.5 * K
@@ -206,11 +222,14 @@ class RHF:
             P = self._density(C)
         if not converged:
             raise RuntimeError(f"SCF failed to converge after {cfg.max_iterations} steps")
+        if mixed:
+            # converged before the switch point: the final rebuild is FP64
+            self.eri.set_precision(False)
         # final full rebuild (scf_impl.cpp:612-621)
         J, K = self._jk(P)
         F = self.H + J - 0.5 * K
         E = self.energy(P, F)
         e, C = self._diag(F)
         self.result = dict(energy=E, P=P, F=F, C=C, eps=e, iterations=step + 1, converged=converged,
-                           build_seconds=float(np.sum(self.build_times)))
+                           build_seconds=float(np.sum(self.build_times)), mixed_builds=self.mixed_builds)
         return self.result
