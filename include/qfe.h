@@ -94,6 +94,14 @@ int qfe_build_jk(qfe_handle* h, const double* P, double* J, double* K, double al
 int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK, double alpha,
                         double beta, double omega, void* stream);
 
+/* Engine-owned HBM buffers of nbf x nbf doubles for callers that keep P, J
+ * and K resident on the device (e.g. to time qfe_build_jk_device without
+ * PCIe traffic), and a synchronous copy on the engine's stream
+ * (kind 0: host -> device, 1: device -> host).  No reference counterpart:
+ * the reference's build_JK takes host pointers only. */
+int qfe_device_buffers(qfe_handle* h, double** dP, double** dJ, double** dK);
+int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes, int kind);
+
 /* ERI nuclear-gradient contributions dJ, dK (3*natoms, xyz-major) for one
  * density.  Replaces ERI::get_gradients (eri.h:100-101;
  * libint2_direct.cpp:646-890). */

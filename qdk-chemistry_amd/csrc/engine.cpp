@@ -43,7 +43,7 @@ struct Unsupported : std::runtime_error {
   do {                                                                                     \
     hipError_t e_ = (x);                                                                   \
     if (e_ != hipSuccess)                                                                  \
-      throw DeviceError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);  \
+      throw ::qfe::DeviceError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);  \
   } while (0)
 
 template <typename T>
@@ -704,6 +704,32 @@ int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK,
     if (std::fabs(omega) > 1e-12)
       throw qfe::Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
     h->e.build_device(dP, dJ, dK, alpha + beta, stream ? (hipStream_t)stream : h->e.stream);
+  });
+}
+
+int qfe_device_buffers(qfe_handle* h, double** dP, double** dJ, double** dK) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    auto& e = h->e;
+    HIP_CHECK(hipSetDevice(e.device));
+    const size_t nn = (size_t)e.nbf * e.nbf;
+    e.P_d.alloc(nn);
+    e.J_d.alloc(nn);
+    e.K_d.alloc(nn);
+    if (dP) *dP = e.P_d.p;
+    if (dJ) *dJ = e.J_d.p;
+    if (dK) *dK = e.K_d.p;
+  });
+}
+
+int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes, int kind) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    if (kind != 0 && kind != 1) throw std::invalid_argument("kind must be 0 (H2D) or 1 (D2H)");
+    HIP_CHECK(hipSetDevice(h->e.device));
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                             h->e.stream));
+    HIP_CHECK(hipStreamSynchronize(h->e.stream));
   });
 }
 

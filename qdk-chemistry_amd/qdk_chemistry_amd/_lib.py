@@ -40,7 +40,7 @@ class QfeStats(ctypes.Structure):
 
 
 EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients", "qfe_one_body",
-           "qfe_one_body_grad", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_nbf", "qfe_destroy", "qfe_last_error"]
+           "qfe_one_body_grad", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_destroy", "qfe_last_error"]
 
 _lib = None
 
@@ -70,6 +70,9 @@ def lib():
         L.qfe_one_body_grad.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, _dp]
         L.qfe_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(QfeStats)]
         L.qfe_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.qfe_device_buffers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+        L.qfe_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int]
         L.qfe_class_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp]
         L.qfe_nbf.argtypes = [ctypes.c_void_p]
         L.qfe_destroy.argtypes = [ctypes.c_void_p]

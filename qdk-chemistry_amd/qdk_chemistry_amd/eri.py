@@ -80,12 +80,29 @@ class RysERI:
         _lib.check(_lib.lib().qfe_build_jk(self.h, _lib.ptr(P), _lib.ptr(J), _lib.ptr(K), alpha, beta, omega, ndm))
         return J, K
 
+    def device_buffers(self):
+        """Engine-owned HBM buffers (P, J, K) as raw device addresses."""
+        p, j, k = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(_lib.lib().qfe_device_buffers(self.h, ctypes.byref(p), ctypes.byref(j), ctypes.byref(k)))
+        return p.value, j.value, k.value
+
+    def upload(self, dst, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        _lib.check(_lib.lib().qfe_copy(self.h, ctypes.c_void_p(dst), host.ctypes.data_as(ctypes.c_void_p),
+                                       host.nbytes, 0))
+
+    def download(self, src, shape):
+        out = np.empty(shape, dtype=np.float64)
+        _lib.check(_lib.lib().qfe_copy(self.h, out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(src),
+                                       out.nbytes, 1))
+        return out
+
     def build_JK_device(self, dP, dJ, dK, alpha=1.0, beta=0.0, omega=0.0, stream=None):
-        """Device-resident variant: dP, dJ, dK are torch CUDA (HIP) tensors."""
+        """Device-resident variant on raw HIP device addresses (ints)."""
         s = ctypes.c_void_p(stream) if stream is not None else None
-        _lib.check(_lib.lib().qfe_build_jk_device(self.h, ctypes.c_void_p(dP.data_ptr()),
-                                                  ctypes.c_void_p(dJ.data_ptr()) if dJ is not None else None,
-                                                  ctypes.c_void_p(dK.data_ptr()) if dK is not None else None,
+        _lib.check(_lib.lib().qfe_build_jk_device(self.h, ctypes.c_void_p(dP),
+                                                  ctypes.c_void_p(dJ) if dJ is not None else None,
+                                                  ctypes.c_void_p(dK) if dK is not None else None,
                                                   alpha, beta, omega, s))
 
     def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0):

@@ -82,7 +82,6 @@ def test_range_separated_exchange_unsupported(water):
 
 
 def test_device_entry_matches_host_entry(water):
-    import torch
     from qdk_chemistry_amd.basis import load_basis
     from qdk_chemistry_amd.eri import create
     from conftest import random_density
@@ -90,12 +89,12 @@ def test_device_entry_matches_host_entry(water):
     eng = create(b, water)
     P = random_density(b.nbf, 12)
     J, K = eng.build_JK(P)
-    dP = torch.from_numpy(P).cuda()
-    dJ, dK = torch.empty_like(dP), torch.empty_like(dP)
+    dP, dJ, dK = eng.device_buffers()
+    eng.upload(dP, P)
     eng.build_JK_device(dP, dJ, dK)
-    torch.cuda.synchronize()
-    assert np.max(np.abs(dJ.cpu().numpy() - J)) < 1e-13
-    assert np.max(np.abs(dK.cpu().numpy() - K)) < 1e-13
+    n = b.nbf
+    assert np.max(np.abs(eng.download(dJ, (n, n)) - J)) < 1e-13
+    assert np.max(np.abs(eng.download(dK, (n, n)) - K)) < 1e-13
 
 
 def test_mixed_precision_scf_energy():
