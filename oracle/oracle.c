@@ -1270,7 +1270,13 @@ int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK,
             for (int k = 0; k < n3; ++k)
               for (int l = 0; l < n4; ++l, ++x) {
                 vj += P[(size_t)(b1 + i) * n + b2 + j] * P[(size_t)(b3 + k) * n + b4 + l] * sh[x];
-                vk += P[(size_t)(b1 + i) * n + b3 + k] * P[(size_t)(b2 + j) * n + b4 + l] * ks * sh[x];
+                /* ordered loop: P_ik P_jl; canonical loop: the permutation
+                 * average (P_ik P_jl + P_il P_jk) / 2 times the degeneracy */
+                const double pk = (mode == 0)
+                                      ? P[(size_t)(b1 + i) * n + b3 + k] * P[(size_t)(b2 + j) * n + b4 + l]
+                                      : 0.5 * (P[(size_t)(b1 + i) * n + b3 + k] * P[(size_t)(b2 + j) * n + b4 + l] +
+                                               P[(size_t)(b1 + i) * n + b4 + l] * P[(size_t)(b2 + j) * n + b3 + k]);
+                vk += pk * ks * sh[x];
               }
         int coord = atoms[d / 3] + (d % 3) * na;
         lJ[coord] += 0.5 * deg * vj;
@@ -1286,10 +1292,6 @@ int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK,
     free(lK);
     free(buf);
     work_free(&w);
-  }
-  if (mode == 1) {
-    /* canonical quartets with weights cover the ordered sum only when P is
-     * symmetric; (ab|cd) pairs were enumerated once per unordered pair. */
   }
   if (dJ) memcpy(dJ, gJ, sizeof(double) * 3 * na);
   if (dK) memcpy(dK, gK, sizeof(double) * 3 * na);

@@ -197,6 +197,9 @@ struct Engine {
   DevBuf<int2> psh_d;  // (sa, sb) per sorted pair, for the screening scan
   DevBuf<double4> ppos_d;
   DevBuf<double> pk_d;
+  DevBuf<double> pa_d;      // exponent of each primitive pair's first shell (gradients)
+  DevBuf<int> sh_atom_d;
+  DevBuf<double> grad_d;    // [2][3 * natoms]
   DevBuf<float4> ppos32_d;  // single-precision copies for the FP32 tasks
   DevBuf<float> pk32_d;
   DevBuf<double> Q_d;
@@ -317,7 +320,7 @@ struct Engine {
     // primitive pairs
     std::vector<PairHdr> hdr(raw.size());
     std::vector<double4> ppos;
-    std::vector<double> pk;
+    std::vector<double> pk, pa;
     for (size_t i = 0; i < raw.size(); ++i) {
       const auto& A = basis.shells[raw[i].s1];
       const auto& B = basis.shells[raw[i].s2];
@@ -343,6 +346,7 @@ struct Engine {
           ppos.push_back(make_double4(p, (a * A.O[0] + b * B.O[0]) / p, (a * A.O[1] + b * B.O[1]) / p,
                                       (a * A.O[2] + b * B.O[2]) / p));
           pk.push_back(K / p);
+          pa.push_back(a);
         }
       h.nprim = (int)ppos.size() - h.prim0;
       h.pad = 0;
@@ -386,7 +390,7 @@ struct Engine {
     pairs_h.resize(npair);
     Q_h.resize(npair);
     std::vector<double4> ppos2;
-    std::vector<double> pk2;
+    std::vector<double> pk2, pa2;
     ppos2.reserve(ppos.size());
     pk2.reserve(pk.size());
     for (int i = 0; i < npair; ++i) {
@@ -396,6 +400,7 @@ struct Engine {
       for (int k = 0; k < h.nprim; ++k) {
         ppos2.push_back(ppos[old0 + k]);
         pk2.push_back(pk[old0 + k]);
+        pa2.push_back(pa[old0 + k]);
       }
       pairs_h[i] = h;
       Q_h[i] = Q[ord[i]];
@@ -443,6 +448,7 @@ struct Engine {
     psh_d.upload(psh, stream);
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
+    pa_d.upload(pa2, stream);
     {
       std::vector<float4> p32(ppos2.size());
       for (size_t i = 0; i < ppos2.size(); ++i)
@@ -514,6 +520,9 @@ struct Engine {
     }
     sh_bf0_d.upload(bf0, stream);
     sh_nf_d.upload(nf, stream);
+    std::vector<int> sha(nsh);
+    for (int s2 = 0; s2 < nsh; ++s2) sha[s2] = basis.shells[s2].atom;
+    sh_atom_d.upload(sha, stream);
     Pn_d.alloc((size_t)nsh * nsh);
     Pnd_d.alloc((size_t)nsh * nsh);
     pmax_d.alloc(1);
@@ -527,17 +536,16 @@ struct Engine {
     }
   }
 
-  // J/K for one density on device buffers
-  void build_device(const double* dP, double* dJ, double* dK, double kscale, hipStream_t s) {
-    HIP_CHECK(hipSetDevice(device));
-    const size_t nn = (size_t)nbf * nbf;
-    HIP_CHECK(hipEventRecord(ev[0], s));
+  // Screened canonical task list for density dP (FP64 and FP32 parts);
+  // returns the class-pair boundaries [2][ncp + 1] of the task array.
+  std::vector<long long> screen(const double* dP, hipStream_t s, bool allow_mix) {
     HIP_CHECK(hipMemsetAsync(bad_d.p, 0, sizeof(int), s));
     launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, Pnd_d.p, bad_d.p, pmax_d.p, s);
     HIP_CHECK(hipGetLastError());
     const long long nitems = (long long)items_h.size();
+    const double mthr = (mix && allow_mix) ? mix_thr : -1.0;
     launch_task_count(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, mix ? mix_thr : -1.0, counts_d.p, s);
+                      pmax_d.p, eri_thr, mthr, counts_d.p, s);
     HIP_CHECK(hipGetLastError());
     launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, 2 * nitems + 1, s);
     HIP_CHECK(hipGetLastError());
@@ -551,8 +559,8 @@ struct Engine {
     launch_gather_offsets(offsets_d.p, bidx_d.p, 2 * (ncp + 1), bval_d.p, s);
     std::vector<long long> bval(2 * (ncp + 1));
     int bad = 0;
-    HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * 2 * (ncp + 1), hipMemcpyDeviceToHost, s));
     long long total = 0;
+    HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * 2 * (ncp + 1), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&total, offsets_d.p + 2 * nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&bad, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(cstat_h.data(), cstat_d.p, sizeof(double) * 4 * ncp, hipMemcpyDeviceToHost, s));
@@ -560,8 +568,57 @@ struct Engine {
     if (bad) throw std::runtime_error("Density matrix contains NaN/Inf values.");
     if (total > 0) tasks_d.alloc((size_t)total);
     launch_task_write(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, mix ? mix_thr : -1.0, counts_d.p, offsets_d.p, tasks_d.p, s);
+                      pmax_d.p, eri_thr, mthr, counts_d.p, offsets_d.p, tasks_d.p, s);
     HIP_CHECK(hipGetLastError());
+    return bval;
+  }
+
+  // ERI gradient contributions (libint2_direct.cpp:646-890) for one density
+  void gradients(const double* P, double* dJ, double* dK, double kscale) {
+    HIP_CHECK(hipSetDevice(device));
+    const size_t nn = (size_t)nbf * nbf;
+    P_d.alloc(nn);
+    grad_d.alloc(6 * (size_t)natoms);
+    hipStream_t s = stream;
+    HIP_CHECK(hipMemcpyAsync(P_d.p, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemsetAsync(grad_d.p, 0, sizeof(double) * 6 * natoms, s));
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    std::vector<long long> bval = screen(P_d.p, s, false);
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    const int ncp = (int)cp_bra.size();
+    long long nq = 0;
+    for (int cp = 0; cp < ncp; ++cp) {
+      const long long n = bval[cp + 1] - bval[cp];
+      if (n <= 0) continue;
+      nq += n;
+      GradLaunch a{tasks_d.p + bval[cp], n, pairs_d.p, ppos_d.p, pk_d.p, pa_d.p, &ra, P_d.p, nbf,
+                   sh_atom_d.p, natoms, kscale, grad_d.p, grad_d.p + 3 * natoms};
+      launch_grad(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
+      HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipEventRecord(ev[2], s));
+    std::vector<double> g(6 * (size_t)natoms);
+    HIP_CHECK(hipMemcpyAsync(g.data(), grad_d.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (dJ) std::memcpy(dJ, g.data(), sizeof(double) * 3 * natoms);
+    if (dK) std::memcpy(dK, g.data() + 3 * natoms, sizeof(double) * 3 * natoms);
+    float t01 = 0, t12 = 0;
+    HIP_CHECK(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    stats.quartets = nq;
+    stats.quartets_fp32 = 0;
+    stats.ms_screen = t01;
+    stats.ms_eri = t12;
+    stats.ms_total = t01 + t12;
+  }
+
+  // J/K for one density on device buffers
+  void build_device(const double* dP, double* dJ, double* dK, double kscale, hipStream_t s) {
+    HIP_CHECK(hipSetDevice(device));
+    const size_t nn = (size_t)nbf * nbf;
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    const int ncp = (int)cp_bra.size();
+    std::vector<long long> bval = screen(dP, s, true);
     HIP_CHECK(hipEventRecord(ev[1], s));
     if (dJ) HIP_CHECK(hipMemsetAsync(dJ, 0, nn * sizeof(double), s));
     if (dK) HIP_CHECK(hipMemsetAsync(dK, 0, nn * sizeof(double), s));
@@ -736,14 +793,11 @@ int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes
 int qfe_gradients(qfe_handle* h, const double* P, double* dJ, double* dK, double alpha, double beta,
                   double omega) {
   return guarded([&] {
-    (void)h;
-    (void)P;
-    (void)dJ;
-    (void)dK;
-    (void)alpha;
-    (void)beta;
-    (void)omega;
-    throw qfe::Unsupported("nuclear gradients are not available yet");
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    if (!P) throw std::invalid_argument("null density");
+    if (std::fabs(omega) > 1e-12)
+      throw qfe::Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
+    h->e.gradients(P, dJ, dK, alpha + beta);
   });
 }
 

@@ -234,15 +234,19 @@ struct Cls {
 };
 
 // 1-D integrals of one direction for one root: I[a][b][c][d]
-template <class C, typename R>
-struct OneD {
-  R v[C::LA + 1][C::LB + 1][C::LC + 1][C::LD + 1];
+template <int AMX, int BMX, int CMX, int DMX, typename R>
+struct OneDX {
+  R v[AMX + 1][BMX + 1][CMX + 1][DMX + 1];
 };
-
-// 2-D recursion + 1-D horizontal transfer for one direction
 template <class C, typename R>
-QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, R>& out) {
-  constexpr int EA = C::EA, FC = C::FC;
+using OneD = OneDX<C::LA, C::LB, C::LC, C::LD, R>;
+
+// 2-D recursion G(i, k), i <= EA, k <= FC, then 1-D horizontal transfer to
+// I(a, b, c, d) for a <= AMX, b <= BMX, c <= CMX, d <= DMX with a + b <= EA
+// and c + d <= FC (entries outside those bounds are not written).
+template <int EA, int FC, int AMX, int BMX, int CMX, int DMX, typename R>
+QFE_DI void rys_1d_x(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD,
+                     OneDX<AMX, BMX, CMX, DMX, R>& out) {
   R g[EA + 1][FC + 1];
   g[0][0] = s;
   if constexpr (EA >= 1) g[1][0] = C00 * s;
@@ -265,30 +269,44 @@ QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, 
       });
     });
   }
-  // ket transfer: h[i][c][d] for i <= EA, from g[i][c + d]
-  R h[EA + 1][C::LC + 1][C::LD + 1];
+  // ket transfer: h[i][c][d] from g[i][c + d]: (c, d) = (c+1, d-1) + CD (c, d-1)
+  R h[EA + 1][CMX + 1][DMX + 1];
   sfor<EA + 1>([&](auto i) {
-    // (c, d) = (c+1, d-1) + CD (c, d-1); needs c + d <= FC
-    R t[FC + 1][C::LD + 1];
+    R t[FC + 1][DMX + 1];
     sfor<FC + 1>([&](auto k) { t[k][0] = g[i][k]; });
-    sfor<C::LD>([&](auto dd) {
+    sfor<DMX>([&](auto dd) {
       constexpr int d = dd + 1;
       sfor<FC + 1 - d>([&](auto k) { t[k][d] = t[k + 1][d - 1] + CD * t[k][d - 1]; });
     });
-    sfor<C::LC + 1>([&](auto c) { sfor<C::LD + 1>([&](auto d) { h[i][c][d] = t[c][d]; }); });
-  });
-  // bra transfer: I[a][b][c][d] from h[a + b][c][d]
-  sfor<C::LC + 1>([&](auto c) {
-    sfor<C::LD + 1>([&](auto d) {
-      R t[EA + 1][C::LB + 1];
-      sfor<EA + 1>([&](auto i) { t[i][0] = h[i][c][d]; });
-      sfor<C::LB>([&](auto bb) {
-        constexpr int b = bb + 1;
-        sfor<EA + 1 - b>([&](auto i) { t[i][b] = t[i + 1][b - 1] + AB * t[i][b - 1]; });
+    sfor<CMX + 1>([&](auto c) {
+      sfor<DMX + 1>([&](auto d) {
+        if constexpr (c + d <= FC) h[i][c][d] = t[c][d];
       });
-      sfor<C::LA + 1>([&](auto a) { sfor<C::LB + 1>([&](auto b) { out.v[a][b][c][d] = t[a][b]; }); });
     });
   });
+  // bra transfer: I[a][b][c][d] from h[a + b][c][d]
+  sfor<CMX + 1>([&](auto c) {
+    sfor<DMX + 1>([&](auto d) {
+      if constexpr (c + d <= FC) {
+        R t[EA + 1][BMX + 1];
+        sfor<EA + 1>([&](auto i) { t[i][0] = h[i][c][d]; });
+        sfor<BMX>([&](auto bb) {
+          constexpr int b = bb + 1;
+          sfor<EA + 1 - b>([&](auto i) { t[i][b] = t[i + 1][b - 1] + AB * t[i][b - 1]; });
+        });
+        sfor<AMX + 1>([&](auto a) {
+          sfor<BMX + 1>([&](auto b) {
+            if constexpr (a + b <= EA) out.v[a][b][c][d] = t[a][b];
+          });
+        });
+      }
+    });
+  });
+}
+
+template <class C, typename R>
+QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, R>& out) {
+  rys_1d_x<C::EA, C::FC, C::LA, C::LB, C::LC, C::LD, R>(s, C00, C00p, B00, B10, B01, AB, CD, out);
 }
 
 // Contract all primitive quartets of (bra|ket) into acc[q][kc] for the bra

@@ -33,6 +33,23 @@ struct JKLaunch {
   const RysArgs32* ra32;
 };
 
+struct GradLaunch {
+  const int2* tasks;
+  long long ntask;
+  const PairHdr* pairs;
+  const double4* ppos;
+  const double* pk;
+  const double* pa;       // exponent of each primitive pair's first shell
+  const RysArgs* ra;
+  const double* P;
+  int nbf;
+  const int* sh_atom;
+  int natoms;
+  double kscale;
+  double* gJ;             // 3 * natoms, xyz-major
+  double* gK;
+};
+
 struct SchwarzLaunch {
   const int* plist;
   int npair;
@@ -45,6 +62,7 @@ struct SchwarzLaunch {
 
 void launch_jk(int bra_cls, int ket_cls, bool pure, const JKLaunch& a, hipStream_t s);
 void launch_schwarz(int pair_cls, bool pure, const SchwarzLaunch& a, hipStream_t s);
+void launch_grad(int bra_cls, int ket_cls, bool pure, const GradLaunch& a, hipStream_t s);
 void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
                        float* Pn, double* Pnd, int* bad, double* pmax, hipStream_t s);
 void launch_task_count(const int2* items, long long nitems, const int2* buckets,

@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .eri import ERIConfig, create, one_body
+from .eri import ERIConfig, create, one_body, one_body_grad
 
 
 @dataclass
@@ -233,3 +233,24 @@ class RHF:
         self.result = dict(energy=E, P=P, F=F, C=C, eps=e, iterations=step + 1, converged=converged,
                            build_seconds=float(np.sum(self.build_times)), mixed_builds=self.mixed_builds)
         return self.result
+
+    def gradient(self):
+        """Nuclear gradient of the converged RHF energy, natoms x 3 (hartree/bohr).
+
+        Restates SCFImpl::get_gradients_ (scf_impl.cpp:726-843) for RHF:
+        nuclear repulsion + sum P (T' + V') - sum W S' + dJ + dK, with the
+        energy-weighted density W = 2 sum_occ eps_i C_i C_i^T (= 1/2 P F P at
+        convergence) and dJ, dK from the adjoint ERI kernel."""
+        if not hasattr(self, "result"):
+            raise RuntimeError("run() the SCF before asking for its gradient")
+        r = self.result
+        P, C, eps = r["P"], r["C"], r["eps"]
+        Co = C[:, : self.nocc]
+        W = 2.0 * (Co * eps[: self.nocc]) @ Co.T
+        b, m = self.basis, self.structure
+        g = m.nuclear_repulsion_gradient()
+        g = g + one_body_grad(b, m, 1, P) + one_body_grad(b, m, 2, P) - one_body_grad(b, m, 0, W)
+        dJ, dK = self.eri.get_gradients(P, 1.0, 0.0, 0.0)
+        na = m.natoms
+        self.grad_parts = dict(dJ=dJ.reshape(3, na).T.copy(), dK=dK.reshape(3, na).T.copy())
+        return g + (dJ + dK).reshape(3, na).T

@@ -1,0 +1,41 @@
+"""Copy the reference's RHF gradient fixtures into reference_fixtures.json.
+
+Sources (data only): scf/tests/data/h2o.json and water10.json (RHF/HF/6-31g*/
+pure energy + gradient, atom-major), geometries from scf/tests/test_common.h
+(make_h2o, make_water10; angstrom with the local bohr factor 0.52917721092).
+Run in a checkout that holds the reference; the JSON output is committed.
+"""
+import json
+import os
+import re
+
+ROOT = "/root/reference/cpp/src/qdk/chemistry/algorithms/microsoft/scf/tests"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def coords_from_header(fn_name):
+    src = open(os.path.join(ROOT, "test_common.h")).read()
+    body = src[src.index(f"inline std::shared_ptr<Molecule> {fn_name}()"):]
+    body = body[: body.index("return mol;")]
+    nums = re.search(r"atomic_nums = \{([^}]*)\}", body).group(1)
+    Z = [int(x) for x in nums.replace("\n", " ").split(",") if x.strip()]
+    block = body[body.index("mol->coords = {"):]
+    triples = re.findall(r"\{\s*([-\d.]+),\s*([-\d.]+),\s*([-\d.]+)\s*\}", block)
+    return Z, [[float(a), float(b), float(c)] for a, b, c in triples[: len(Z)]]
+
+
+def main():
+    out = {"bohr_to_angstrom": 0.52917721092}
+    for name, fn, data in (("h2o", "make_h2o", "h2o.json"), ("water10", "make_water10", "water10.json")):
+        Z, R = coords_from_header(fn)
+        rec = json.load(open(os.path.join(ROOT, "data", data)))["RHF/HF/6-31g*/pure"]
+        grad = rec["gradient"]
+        if isinstance(grad, str):
+            grad = json.loads(grad)
+        out[name] = dict(numbers=Z, coords_angstrom=R, basis="6-31g*", pure=True,
+                         energy=float(rec["energy"]), gradient=[float(g) for g in grad])
+    json.dump(out, open(os.path.join(HERE, "reference_fixtures.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

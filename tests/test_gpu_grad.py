@@ -1,0 +1,108 @@
+"""Nuclear gradients: adjoint kernel vs the oracle's restatement of
+ERI::get_gradients, the RHF gradient vs the reference's fixtures
+(scf/tests/data/h2o.json, water10.json) and central finite differences."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import random_density
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _setup(mol, basis="cc-pvdz"):
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import create
+    b = load_basis(basis, mol.numbers, mol.coords)
+    return b, create(b, mol)
+
+
+def test_eri_gradient_vs_oracle_reference_loop(water):
+    from oracle.oracle import OracleERI
+    b, eng = _setup(water)
+    P = random_density(b.nbf, 21)
+    dJ, dK = eng.get_gradients(P)
+    rJ, rK = OracleERI(b, water).get_gradients(P, mode=0)  # the reference's ordered loop
+    print("max|ddJ| %.2e max|ddK| %.2e" % (np.max(np.abs(dJ - rJ)), np.max(np.abs(dK - rK))), flush=True)
+    assert np.max(np.abs(dJ - rJ)) < 1e-10
+    assert np.max(np.abs(dK - rK)) < 1e-10
+
+
+def test_eri_gradient_vs_oracle_cluster():
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 11)
+    b, eng = _setup(mol)
+    P = random_density(b.nbf, 22, 0.3)
+    dJ, dK = eng.get_gradients(P)
+    rJ, rK = OracleERI(b, mol).get_gradients(P, mode=1)
+    assert np.max(np.abs(dJ - rJ)) < 1e-10
+    assert np.max(np.abs(dK - rK)) < 1e-10
+
+
+def _fixture_mol(name):
+    from qdk_chemistry_amd.molecule import Structure
+    d = json.load(open(os.path.join(GOLD, "reference_fixtures.json")))
+    rec = d[name]
+    R = np.array(rec["coords_angstrom"]) / d["bohr_to_angstrom"]
+    return Structure(np.array(rec["numbers"]), R), rec
+
+
+@pytest.mark.parametrize("name,etol,gtol", [("h2o", 1e-8, 1e-6), ("water10", 1e-7, 1e-6)])
+def test_rhf_gradient_vs_reference_fixture(name, etol, gtol):
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.scf import RHF, SCFConfig
+    mol, rec = _fixture_mol(name)
+    b = load_basis(rec["basis"], mol.numbers, mol.coords, pure=rec["pure"])
+    scf = RHF(mol, b, SCFConfig(convergence_threshold=1e-9))
+    r = scf.run()
+    g = scf.gradient()
+    ref = np.array(rec["gradient"]).reshape(-1, 3)
+    print(name, "dE %.2e max|dg| %.2e" % (r["energy"] - rec["energy"], np.max(np.abs(g - ref))), flush=True)
+    assert abs(r["energy"] - rec["energy"]) < etol
+    assert np.max(np.abs(g - ref)) < gtol
+    assert np.max(np.abs(g.sum(axis=0))) < 1e-8  # translational invariance
+
+
+def test_rhf_gradient_vs_central_differences(water):
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.molecule import Structure
+    from qdk_chemistry_amd.scf import RHF, SCFConfig
+    cfg = SCFConfig(convergence_threshold=1e-10)
+    b = load_basis("cc-pvdz", water.numbers, water.coords)
+    scf = RHF(water, b, cfg)
+    scf.run()
+    g = scf.gradient()
+    h = 1e-4
+    worst = 0.0
+    for a in range(water.natoms):
+        for x in range(3):
+            e = []
+            for sgn in (1.0, -1.0):
+                R = water.coords.copy()
+                R[a, x] += sgn * h
+                m = Structure(water.numbers, R)
+                e.append(RHF(m, load_basis("cc-pvdz", m.numbers, m.coords), cfg).run()["energy"])
+            worst = max(worst, abs((e[0] - e[1]) / (2 * h) - g[a, x]))
+    print("max |g - fd| = %.2e" % worst, flush=True)
+    assert worst <= 1e-6
+
+
+def test_autograd_eri_energy(water):
+    import torch
+    from qdk_chemistry_amd.autograd import ERIEnergy
+    b, _ = _setup(water)
+    P = torch.from_numpy(random_density(b.nbf, 3, 0.2))
+    R = torch.from_numpy(water.coords.copy()).requires_grad_(True)
+    E = ERIEnergy.apply(R, P, water.numbers, "cc-pvdz")
+    E.backward()
+    h = 1e-4
+    for a, x in ((0, 2), (1, 0), (2, 1)):
+        Rp, Rm = R.detach().clone(), R.detach().clone()
+        Rp[a, x] += h
+        Rm[a, x] -= h
+        fd = (ERIEnergy.apply(Rp, P, water.numbers, "cc-pvdz") - ERIEnergy.apply(Rm, P, water.numbers, "cc-pvdz")) / (2 * h)
+        assert abs(float(fd) - float(R.grad[a, x])) < 1e-6
