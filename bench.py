@@ -103,6 +103,26 @@ def summarize(run, steps):
                 dominant=dom)
 
 
+def log_classes(run, steps, top=12):
+    """Per-class table on stderr: quartets, ms and TFLOP/s per build, and the
+    number of global FP64 atomics the digestion issues (J_cd + four K blocks,
+    J_ab wave-reduced and not counted)."""
+    from qdk_chemistry_amd.flops import class_flops, nsph
+    rows = []
+    for (prec, cls), (nq, npq, ms, n) in run["per_class"].items():
+        fp, fc = class_flops(*cls)
+        na, nb, nc, nd = (nsph(x) for x in cls)
+        atom = nq * (nc * nd + na * nc + na * nd + nb * nc + nb * nd)
+        rows.append((ms / steps, prec, cls, nq / steps, npq / steps, (npq * fp + nq * fc) / (ms * 1e-3) / 1e12,
+                     atom / steps))
+    rows.sort(reverse=True)
+    tot_at = sum(r[6] for r in rows)
+    log(f"classes by time (of {len(rows)}); global FP64 atomics per build {tot_at:.3g}")
+    for ms, prec, cls, nq, npq, tf, at in rows[:top]:
+        log(f"  {'f32' if prec else 'f64'} ({cls[0]}{cls[1]}|{cls[2]}{cls[3]}) {ms:8.2f} ms  quartets {nq:.3g}  "
+            f"prim {npq:.3g}  {tf:6.2f} TFLOP/s  atomics {at:.3g}")
+
+
 def cpu_baseline(basis, mol, P, stride):
     from oracle.oracle import OracleERI
     threads = min(16, os.cpu_count() or 1)
@@ -169,6 +189,7 @@ def main():
     log(f"FP64 timed: {args.steps} builds, {s64['ms_per_build']:.2f} ms/build, {s64['quartets_per_s']:.4g} quartets/s, "
         f"ERI kernels {s64['eri_kernels_ms']:.2f} ms ({s64['peak_fraction_eri']:.3f} of peak), screen "
         f"{s64['screen_ms']:.2f} ms; dominant class {cls}: {ms_launch:.2f} ms, {achieved / 1e12:.2f} TFLOP/s")
+    log_classes(run64, args.steps)
     headline = s64
     path = "fp64"
     mixed = None

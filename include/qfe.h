@@ -71,9 +71,6 @@ typedef struct {
   long long quartets;          /* shell quartets that passed screening */
   long long quartets_fp32;     /* of which evaluated in FP32 */
   long long prim_quartets;     /* primitive quartets of the evaluated shell quartets */
-  long long func_quartets;     /* function quartets */
-  double flops_fp64;           /* algorithmic FP64 flops (see DESIGN) */
-  double flops_fp32;
   double ms_screen;            /* task-list construction */
   double ms_eri;               /* ERI + digestion kernels */
   double ms_total;             /* whole build, device time */

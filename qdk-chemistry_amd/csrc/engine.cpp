@@ -575,6 +575,7 @@ struct Engine {
 
   // ERI gradient contributions (libint2_direct.cpp:646-890) for one density
   void gradients(const double* P, double* dJ, double* dK, double kscale) {
+    if (natoms <= 0) throw std::invalid_argument("gradients need the atoms the shells sit on");
     HIP_CHECK(hipSetDevice(device));
     const size_t nn = (size_t)nbf * nbf;
     P_d.alloc(nn);

@@ -33,8 +33,7 @@ class QfeConfig(ctypes.Structure):
 
 class QfeStats(ctypes.Structure):
     _fields_ = [("quartets", ctypes.c_longlong), ("quartets_fp32", ctypes.c_longlong),
-                ("prim_quartets", ctypes.c_longlong), ("func_quartets", ctypes.c_longlong),
-                ("flops_fp64", ctypes.c_double), ("flops_fp32", ctypes.c_double),
+                ("prim_quartets", ctypes.c_longlong),
                 ("ms_screen", ctypes.c_double), ("ms_eri", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("significant_pairs", ctypes.c_longlong)]
 

@@ -239,14 +239,13 @@ class RHF:
 
         Restates SCFImpl::get_gradients_ (scf_impl.cpp:726-843) for RHF:
         nuclear repulsion + sum P (T' + V') - sum W S' + dJ + dK, with the
-        energy-weighted density W = 2 sum_occ eps_i C_i C_i^T (= 1/2 P F P at
+        energy-weighted density W = 1/2 P F P (= 2 sum_occ eps_i C_i C_i^T at
         convergence) and dJ, dK from the adjoint ERI kernel."""
         if not hasattr(self, "result"):
             raise RuntimeError("run() the SCF before asking for its gradient")
         r = self.result
-        P, C, eps = r["P"], r["C"], r["eps"]
-        Co = C[:, : self.nocc]
-        W = 2.0 * (Co * eps[: self.nocc]) @ Co.T
+        P, F = r["P"], r["F"]
+        W = 0.5 * P @ F @ P  # energy-weighted density (scf_impl.cpp:785-798 holds -W)
         b, m = self.basis, self.structure
         g = m.nuclear_repulsion_gradient()
         g = g + one_body_grad(b, m, 1, P) + one_body_grad(b, m, 2, P) - one_body_grad(b, m, 0, W)

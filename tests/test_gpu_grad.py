@@ -51,7 +51,10 @@ def _fixture_mol(name):
     return Structure(np.array(rec["numbers"]), R), rec
 
 
-@pytest.mark.parametrize("name,etol,gtol", [("h2o", 1e-8, 1e-6), ("water10", 1e-7, 1e-6)])
+# energy: the fixtures agree with the exact energy to <1e-9; gradient: the
+# reference's own tolerance for these records (scf_test.cpp:12, :71), the
+# stored values sit a few 1e-6 from the exact derivative
+@pytest.mark.parametrize("name,etol,gtol", [("h2o", 1e-8, 1e-5), ("water10", 1e-7, 1e-5)])
 def test_rhf_gradient_vs_reference_fixture(name, etol, gtol):
     from qdk_chemistry_amd.basis import load_basis
     from qdk_chemistry_amd.scf import RHF, SCFConfig
