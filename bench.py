@@ -1,6 +1,6 @@
 """Fock-build benchmark: (H2O)64 / cc-pVDZ (1536 basis functions) on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--system h2o64|h2o16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--system h2o64|h2o16|c20h42]
                     [--no-scf] [--no-cpu] [--no-mixed]
 
 One step is one full J/K build (screened task list + ERI class kernels +
@@ -41,6 +41,9 @@ def system(name):
         return water_cluster(64, 20261264), "cc-pvdz", "(H2O)64/cc-pVDZ seed 20261264"
     if name == "h2o16":
         return water_cluster(16, 20261202), "cc-pvdz", "(H2O)16/cc-pVDZ seed 20261202"
+    if name == "c20h42":
+        from qdk_chemistry_amd.molecule import linear_alkane
+        return linear_alkane(20, 20261220), "cc-pvtz", "C20H42/cc-pVTZ seed 20261220"
     raise SystemExit(f"unknown system {name}")
 
 

@@ -94,8 +94,10 @@ int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK,
 /* Engine-owned HBM buffers of nbf x nbf doubles for callers that keep P, J
  * and K resident on the device (e.g. to time qfe_build_jk_device without
  * PCIe traffic), and a synchronous copy on the engine's stream
- * (kind 0: host -> device, 1: device -> host).  No reference counterpart:
- * the reference's build_JK takes host pointers only. */
+ * (kind 0: host -> device, 1: device -> host, 2: device -> device through a
+ * plain copy kernel; kind 2 needs 8-byte aligned pointers and a multiple of 8
+ * bytes).  No reference counterpart: the reference's build_JK takes host
+ * pointers only. */
 int qfe_device_buffers(qfe_handle* h, double** dP, double** dJ, double** dK);
 int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes, int kind);
 
@@ -130,6 +132,9 @@ int qfe_get_stats(const qfe_handle* h, qfe_stats* out);
 int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, double* quartets,
                     double* prim_quartets, double* ms);
 int qfe_nbf(const qfe_handle* h);
+/* Highest shell angular momentum compiled into the ERI kernels of this build
+ * (qfe_create rejects bases with higher l).  No reference counterpart. */
+int qfe_max_l(void);
 void qfe_destroy(qfe_handle* h);
 const char* qfe_last_error(void);
 

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <dlfcn.h>
 #include <fstream>
@@ -27,6 +28,7 @@
 #include "eri_core.hpp"
 #include "host_common.hpp"
 #include "kernels.hpp"
+#include "screen.hpp"
 
 namespace qfe {
 
@@ -190,6 +192,10 @@ struct Engine {
   std::vector<int> pair_bucket_h;
   std::vector<int2> items_h;         // (bra pair, ket bucket)
   std::vector<int> cp_item_begin;    // per class pair: first item (size ncp + 1)
+  std::vector<int2> segs_h;          // item ranges [x, y) of one bra pair within a class pair
+  std::vector<int> cp_seg_begin;     // per class pair: first segment (size ncp + 1)
+  DevBuf<int2> segs_d;
+  int lds_max = 65536;               // dynamic LDS bytes a block may request
   std::vector<int> cp_bra, cp_ket;   // class pair -> (bra class, ket class)
   long long prim_pairs = 0;
 
@@ -427,20 +433,35 @@ struct Engine {
         cp_bra[b * (b + 1) / 2 + k] = b;
         cp_ket[b * (b + 1) / 2 + k] = k;
       }
+    // items bra-pair-major inside each class pair: the items (and so the
+    // tasks) of one bra pair are contiguous, one segment per (bra pair, class pair)
     std::vector<std::vector<int2>> per_cp(ncp);
     for (int bb = 0; bb < nb; ++bb)
-      for (int kb = 0; kb <= bb; ++kb) {
-        const int cb = bucket_cls_h[bb], ck = bucket_cls_h[kb];
-        const int cp = cb * (cb + 1) / 2 + ck;  // cb >= ck since buckets are class-sorted
-        for (int p = buckets_h[bb].x; p < buckets_h[bb].y; ++p) per_cp[cp].push_back(make_int2(p, kb));
-      }
+      for (int p = buckets_h[bb].x; p < buckets_h[bb].y; ++p)
+        for (int kb = 0; kb <= bb; ++kb) {
+          const int cb = bucket_cls_h[bb], ck = bucket_cls_h[kb];
+          const int cp = cb * (cb + 1) / 2 + ck;  // cb >= ck since buckets are class-sorted
+          per_cp[cp].push_back(make_int2(p, kb));
+        }
     items_h.clear();
     cp_item_begin.assign(ncp + 1, 0);
+    segs_h.clear();
+    cp_seg_begin.assign(ncp + 1, 0);
     for (int cp = 0; cp < ncp; ++cp) {
       cp_item_begin[cp] = (int)items_h.size();
-      items_h.insert(items_h.end(), per_cp[cp].begin(), per_cp[cp].end());
+      cp_seg_begin[cp] = (int)segs_h.size();
+      const int base = (int)items_h.size();
+      const std::vector<int2>& v = per_cp[cp];
+      for (size_t i = 0; i < v.size();) {
+        size_t e = i + 1;
+        while (e < v.size() && v[e].x == v[i].x) ++e;
+        segs_h.push_back(make_int2(base + (int)i, base + (int)e));
+        i = e;
+      }
+      items_h.insert(items_h.end(), v.begin(), v.end());
     }
     cp_item_begin[ncp] = (int)items_h.size();
+    cp_seg_begin[ncp] = (int)segs_h.size();
 
     pairs_d.upload(pairs_h, stream);
     std::vector<int2> psh(npair);
@@ -460,6 +481,7 @@ struct Engine {
     buckets_d.upload(buckets_h, stream);
     pair_bucket_d.upload(pair_bucket_h, stream);
     items_d.upload(items_h, stream);
+    segs_d.upload(segs_h, stream);
     std::vector<int> item_cp(items_h.size());
     for (int cp = 0; cp < ncp; ++cp)
       for (int i = cp_item_begin[cp]; i < cp_item_begin[cp + 1]; ++i) item_cp[i] = cp;
@@ -510,6 +532,10 @@ struct Engine {
     HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw std::invalid_argument("device_id out of range");
     HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    // QFE_SEGMENT_DIGEST=0 selects the per-task atomic digestion for every class
+    if (const char* e = std::getenv("QFE_SEGMENT_DIGEST"))
+      if (e[0] == '0') lds_max = 0;
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     setup_rys();
@@ -576,6 +602,8 @@ struct Engine {
   // ERI gradient contributions (libint2_direct.cpp:646-890) for one density
   void gradients(const double* P, double* dJ, double* dK, double kscale) {
     if (natoms <= 0) throw std::invalid_argument("gradients need the atoms the shells sit on");
+    for (const auto& sh : basis.shells)
+      if (sh.l > 2) throw Unsupported("analytic ERI gradients are compiled for s, p and d shells only");
     HIP_CHECK(hipSetDevice(device));
     const size_t nn = (size_t)nbf * nbf;
     P_d.alloc(nn);
@@ -644,7 +672,9 @@ struct Engine {
         if (n <= 0) continue;
         (prec ? nq32 : nq) += n;
         JKLaunch a{tasks_d.p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jw, Kw, nbf, kscale,
-                   prec == 1, ppos32_d.p, pk32_d.p, &ra32};
+                   prec == 1, ppos32_d.p, pk32_d.p, &ra32,
+                   segs_d.p + cp_seg_begin[cp], cp_seg_begin[cp + 1] - cp_seg_begin[cp],
+                   offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_d.p, lds_max};
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
         if (e) HIP_CHECK(hipEventRecord(e[0], s));
         launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
@@ -783,10 +813,18 @@ int qfe_device_buffers(qfe_handle* h, double** dP, double** dJ, double** dK) {
 int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes, int kind) {
   return guarded([&] {
     if (!h) throw std::runtime_error("ERI engine is not initialized");
-    if (kind != 0 && kind != 1) throw std::invalid_argument("kind must be 0 (H2D) or 1 (D2H)");
+    if (kind < 0 || kind > 2) throw std::invalid_argument("kind must be 0 (H2D), 1 (D2H) or 2 (D2D)");
     HIP_CHECK(hipSetDevice(h->e.device));
-    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
-                             h->e.stream));
+    if (kind == 2) {
+      if (bytes % 8 != 0 || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 8 != 0)
+        throw std::invalid_argument("device copies move whole, 8-byte aligned words");
+      qfe::launch_copy_words(static_cast<const unsigned long long*>(src), static_cast<unsigned long long*>(dst),
+                             bytes / 8, h->e.stream);
+      HIP_CHECK(hipGetLastError());
+    } else {
+      HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                               h->e.stream));
+    }
     HIP_CHECK(hipStreamSynchronize(h->e.stream));
   });
 }
@@ -841,6 +879,8 @@ int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, doubl
   }
   return ncp;
 }
+
+int qfe_max_l(void) { return QFE_MAX_L; }
 
 int qfe_nbf(const qfe_handle* h) { return h ? h->e.nbf : -1; }
 

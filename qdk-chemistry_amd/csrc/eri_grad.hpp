@@ -215,14 +215,16 @@ struct GradOps {
   static constexpr int NBC = C::NCA * C::NCB;
   static constexpr int NCH = (NBC + CH - 1) / CH;
 
-  static void run(const GradLaunch& a, hipStream_t s) {
+  template <int C0, int C1, bool LAST>
+  static void run_range(const GradLaunch& a, hipStream_t s) {
+    static_assert(C0 < C1 && C1 <= NCH && (!LAST || C1 == NCH), "chunk range does not match the class");
     if (a.ntask <= 0) return;
     const int block = 256;
     const long long grid = (a.ntask + block - 1) / block;
     const bool lds = a.natoms * 6 * sizeof(double) <= 64 * 1024;
     const size_t shm = lds ? (size_t)a.natoms * 6 * sizeof(double) : 0;
-    hsfor<NCH>([&](auto c) {
-      constexpr int q0 = c * CH;
+    hsfor<C1 - C0>([&](auto cc) {
+      constexpr int q0 = (C0 + cc) * CH;
       constexpr int q1 = (q0 + CH < NBC) ? q0 + CH : NBC;
       if (lds)
         hipLaunchKernelGGL((k_eri_grad<C, q0, q1, true>), dim3((unsigned)grid), dim3(block), shm, s, a.tasks,

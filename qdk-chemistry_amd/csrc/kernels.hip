@@ -11,6 +11,7 @@
 
 #include "eri_core.hpp"
 #include "kernels.hpp"
+#include "screen.hpp"
 
 namespace qfe {
 
@@ -239,6 +240,19 @@ void launch_class_stats(const int* counts, const int2* items, long long nitems, 
 void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
   dim3 block(256), grid((n + 255) / 256, n);
   hipLaunchKernelGGL(k_symmetrize, grid, block, 0, s, J, K, n);
+}
+
+__global__ __launch_bounds__(256) void k_copy_words(const unsigned long long* __restrict__ src,
+                                                    unsigned long long* __restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy_words(const unsigned long long* src, unsigned long long* dst, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const size_t want = (n + 255) / 256;
+  const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace qfe

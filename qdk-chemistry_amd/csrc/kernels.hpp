@@ -1,4 +1,5 @@
-// Host-side declarations of the kernel launchers (implemented in kernels.hip).
+// Launch descriptors and class dispatch of the ERI class kernels (implemented
+// in the generated per-class translation units).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,6 +32,13 @@ struct JKLaunch {
   const float4* ppos32;
   const float* pk32;
   const RysArgs32* ra32;
+  // segment view of the same tasks: segment s covers items [segs[s].x, segs[s].y)
+  // whose tasks are [offs[x], offs[y]) of tasks_all; all share one bra pair
+  const int2* segs;
+  int nseg;
+  const long long* offs;
+  const int2* tasks_all;
+  int lds_max;        // dynamic LDS bytes a block may request (0: flat kernel only)
 };
 
 struct GradLaunch {
@@ -63,24 +71,4 @@ struct SchwarzLaunch {
 void launch_jk(int bra_cls, int ket_cls, bool pure, const JKLaunch& a, hipStream_t s);
 void launch_schwarz(int pair_cls, bool pure, const SchwarzLaunch& a, hipStream_t s);
 void launch_grad(int bra_cls, int ket_cls, bool pure, const GradLaunch& a, hipStream_t s);
-void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
-                       float* Pn, double* Pnd, int* bad, double* pmax, hipStream_t s);
-void launch_task_count(const int2* items, long long nitems, const int2* buckets,
-                       const int* pair_bucket, const int2* pair_shells, const double* Q,
-                       const float* Pn, int nsh, const double* pmax, double thr, double mix_thr,
-                       int* counts, hipStream_t s);
-void launch_task_write(const int2* items, long long nitems, const int2* buckets,
-                       const int* pair_bucket, const int2* pair_shells, const double* Q,
-                       const float* Pn, int nsh, const double* pmax, double thr, double mix_thr,
-                       const int* counts, const long long* offsets, int2* tasks, hipStream_t s);
-size_t scan_temp_bytes(long long n);
-void launch_exclusive_scan(void* temp, size_t temp_bytes, const int* in, long long* out,
-                           long long n, hipStream_t s);
-void launch_gather_offsets(const long long* offsets, const long long* idx, int n, long long* out,
-                           hipStream_t s);
-void launch_class_stats(const int* counts, const int2* items, long long nitems, const int* item_cp,
-                        const int2* buckets, const PairHdr* pairs, int ncp, double* q_out, double* p_out,
-                        hipStream_t s);
-void launch_symmetrize(double* J, double* K, int n, hipStream_t s);
-
 }  // namespace qfe

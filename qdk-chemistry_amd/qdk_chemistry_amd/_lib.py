@@ -39,7 +39,8 @@ class QfeStats(ctypes.Structure):
 
 
 EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients", "qfe_one_body",
-           "qfe_one_body_grad", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_destroy", "qfe_last_error"]
+           "qfe_one_body_grad", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_max_l", "qfe_destroy",
+           "qfe_last_error"]
 
 _lib = None
 

@@ -137,6 +137,14 @@ class RysERI:
         return {k: getattr(s, k) for k, _ in _lib.QfeStats._fields_}
 
 
+def compiled_max_l() -> int:
+    """Highest shell angular momentum the ERI kernels of this build handle."""
+    L = _lib.lib()
+    if not hasattr(L, "qfe_max_l"):  # libraries built before the query existed compile s, p, d
+        return 2
+    return int(L.qfe_max_l())
+
+
 def create(basis, structure, cfg: ERIConfig | None = None):
     """ERI::create (eri.cpp:36-68): dispatch on cfg.method."""
     cfg = cfg or ERIConfig()

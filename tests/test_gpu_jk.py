@@ -63,3 +63,28 @@ def test_mixed_precision_jk_close_to_oracle():
     J, K = eng.build_JK(P)
     assert eng.stats()["quartets_fp32"] == 0
     assert np.max(np.abs(J - Jr)) <= 1e-10 and np.max(np.abs(K - Kr)) <= 1e-10
+
+
+@pytest.mark.parametrize("mix", [False, True])
+def test_segment_and_per_task_digestion_agree(monkeypatch, mix):
+    """The LDS segment digestion (default) and the per-task atomic digestion
+    (QFE_SEGMENT_DIGEST=0) give the same J and K, and both match the oracle."""
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 11)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = random_density(b.nbf, 5, 0.3)
+    cfg = ERIConfig(is_mix_precision=mix)
+    J1, K1 = create(b, mol, cfg).build_JK(P)
+    monkeypatch.setenv("QFE_SEGMENT_DIGEST", "0")
+    J0, K0 = create(b, mol, cfg).build_JK(P)
+    print("segment vs per-task: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
+          flush=True)
+    assert np.max(np.abs(J1 - J0)) < 1e-11
+    assert np.max(np.abs(K1 - K0)) < 1e-11
+    if not mix:
+        Jr, Kr = OracleERI(b, mol).build_JK(P)
+        assert np.max(np.abs(J1 - Jr)) < 1e-10
+        assert np.max(np.abs(K1 - Kr)) < 1e-10

@@ -134,6 +134,8 @@ CRAWFORD = [[0.0, -0.0757918436, 0.0], [0.866811829, 0.6014357793, 0.0], [-0.866
     ("cc-pvdz", "example", -76.02286895, 1e-8),
     # cpp/tests/test_scf.cpp:136-150 (Crawford geometry, cpp/tests/ut_common.hpp:244-260)
     ("def2-svp", "crawford", -75.9229032345009, 1e-8),
+    # cpp/tests/test_scf.cpp:136-150: def2-TZVP puts an f shell on O
+    ("def2-tzvp", "crawford", -76.0205776517675, 1e-8),
     # scf/tests/data/h2o.json RHF/HF/6-31g*/pure, geometry scf/tests/test_common.h:11-32
     ("6-31g*", "h2o_json", -76.00450217800814, 1e-8),
 ])
