@@ -106,6 +106,31 @@ def summarize(run, steps):
                 dominant=dom)
 
 
+def calibration_copy(eng, nbf):
+    """One plain device copy of the density buffer into the J buffer (k_copy_words,
+    8*nbf^2 bytes each way): the reference reading for HBM traffic counters."""
+    from qdk_chemistry_amd import _lib
+    dP, dJ, _ = eng.device_buffers()
+    nbytes = 8 * nbf * nbf
+    _lib.check(_lib.lib().qfe_copy(eng.h, dJ, dP, nbytes, 2))
+    return nbytes
+
+
+def traffic_from_profile(kernel_tag):
+    """HBM bytes per launch of the dominant kernel, from the committed PMC
+    summary (profiles/hbm_traffic.txt: 'tag = bytes' lines written from
+    rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel reading)."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            if k.strip() == kernel_tag:
+                return float(v.split()[0])
+    return None
+
+
 def log_classes(run, steps, top=12):
     """Per-class table on stderr: quartets, ms and TFLOP/s per build, and the
     number of global FP64 atomics the digestion issues (J_cd + four K blocks,
@@ -150,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=24)
+    ap.add_argument("--density", help="load the density from this .npy instead of running the SCF")
+    ap.add_argument("--save-density", help="write the converged density to this .npy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-device "
@@ -171,7 +198,12 @@ def main():
         f"pairs={eng.stats()['significant_pairs']}")
 
     result = {}
-    if args.no_scf:
+    if args.density:
+        P = np.load(args.density)
+        if P.shape != (basis.nbf, basis.nbf):
+            raise SystemExit(f"{args.density}: shape {P.shape} does not match nbf={basis.nbf}")
+        density = f"density loaded from {os.path.basename(args.density)}"
+    elif args.no_scf:
         P = atom_guess(mol, basis_name)
         density = "atomic guess"
     else:
@@ -184,9 +216,12 @@ def main():
                                   fock_build_s=float(np.sum(scf.build_times)), wall_s=time.perf_counter() - t)
         log(f"FP64 SCF: E={r['energy']:.10f} in {r['iterations']} iterations, "
             f"Fock builds {np.sum(scf.build_times):.2f}s")
+        if args.save_density:
+            np.save(args.save_density, P)
 
     run64 = timed_builds(eng, P, args.steps, args.warmup)
     s64 = summarize(run64, args.steps)
+    calib_bytes = calibration_copy(eng, basis.nbf)
     cls, f_launch, nq_launch, ms_launch = s64["dominant"]
     achieved = f_launch / (ms_launch * 1e-3)
     log(f"FP64 timed: {args.steps} builds, {s64['ms_per_build']:.2f} ms/build, {s64['quartets_per_s']:.4g} quartets/s, "
@@ -243,10 +278,12 @@ def main():
         "config": {"workload": f"RHF J/K build, {desc}, nbf={basis.nbf}, eri_threshold 1e-12",
                    "path": path, "fock_build_s": headline["ms_per_build"] / 1e3,
                    "fp64": {k: v for k, v in s64.items() if k != "dominant"}, "mixed": mixed},
-        "roofline": {"bound": "fp64-valu", "kernel": "k_eri_jk class (%d %d|%d %d), FP64" % cls,
+        "roofline": {"bound": "fp64-valu", "kernel": "ERI class (%d %d|%d %d), FP64" % cls,
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK, "traffic": None,
-                     "flops_per_launch": f_launch, "ms_per_launch": ms_launch},
+                     "frac": achieved / FP64_PEAK,
+                     "traffic": traffic_from_profile("%s (%d%d|%d%d) fp64" % (args.system, *cls)),
+                     "flops_per_launch": f_launch, "ms_per_launch": ms_launch,
+                     "calibration_copy_bytes": calib_bytes},
     }
     out["config"].update(result)
     if not args.no_cpu:

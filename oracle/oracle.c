@@ -1204,6 +1204,27 @@ int oracle_jk_incore(const oracle_t *h, const double *P, double *J, double *K, d
  * mode 0: the reference's ordered-pair loop (num_pairs^2, no pair symmetry);
  * mode 1: canonical quartets with degeneracy weights (same sum, 8x fewer).
  */
+/* Restrict oracle_gradients to one ERI class (for locating a faulty class
+ * kernel): key = 1000 la + 100 lb + 10 lc + ld of the class with la >= lb,
+ * lc >= ld and the (la, lb) pair class not below (lc, ld); -1 = all classes. */
+static int g_grad_key = -1;
+void oracle_set_grad_class(int key) { g_grad_key = key; }
+/* With g_grad_centre = X in 0..2 only the derivative by centre X of each
+ * quartet is kept, added to atom(X) and subtracted from atom(D) (the form a
+ * kernel that gets D by translation invariance accumulates); -1 = all. */
+static int g_grad_centre = -1;
+void oracle_set_grad_centre(int x) { g_grad_centre = x; }
+
+static int class_key(int l1, int l2, int l3, int l4) {
+  int a = l1 > l2 ? l1 : l2, b = l1 > l2 ? l2 : l1, c = l3 > l4 ? l3 : l4, d = l3 > l4 ? l4 : l3;
+  int pab = a * (a + 1) / 2 + b, pcd = c * (c + 1) / 2 + d;
+  if (pcd > pab) {
+    int t = a; a = c; c = t;
+    t = b; b = d; d = t;
+  }
+  return 1000 * a + 100 * b + 10 * c + d;
+}
+
 int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK, double alpha,
                      double beta, int mode) {
   const int nsh = h->nsh, n = h->nbf, na = h->natoms;
@@ -1256,6 +1277,7 @@ int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK,
       double pn = max6(Pn[(size_t)s1 * nsh + s2], Pn[(size_t)s1 * nsh + s3], Pn[(size_t)s1 * nsh + s4],
                        Pn[(size_t)s2 * nsh + s3], Pn[(size_t)s2 * nsh + s4], Pn[(size_t)s3 * nsh + s4]);
       if (pn * h->Q[(size_t)s1 * nsh + s2] * h->Q[(size_t)s3 * nsh + s4] < h->eri_thr) continue;
+      if (g_grad_key >= 0 && class_key(h->sh[s1].l, h->sh[s2].l, h->sh[s3].l, h->sh[s4].l) != g_grad_key) continue;
       quartet_out(h, s1, s2, s3, s4, 1, &w, buf);
       int b1 = h->bf0[s1], b2 = h->bf0[s2], b3 = h->bf0[s3], b4 = h->bf0[s4];
       int n1 = h->bf0[s1 + 1] - b1, n2 = h->bf0[s2 + 1] - b2, n3 = h->bf0[s3 + 1] - b3,
@@ -1279,6 +1301,12 @@ int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK,
                 vk += pk * ks * sh[x];
               }
         int coord = atoms[d / 3] + (d % 3) * na;
+        if (g_grad_centre >= 0) {
+          if (d / 3 != g_grad_centre) continue;
+          int cd = atoms[3] + (d % 3) * na;
+          lJ[cd] -= 0.5 * deg * vj;
+          lK[cd] -= -0.25 * deg * vk;
+        }
         lJ[coord] += 0.5 * deg * vj;
         lK[coord] += -0.25 * deg * vk;
       }

@@ -60,6 +60,8 @@ def lib():
         L.oracle_sph_row.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_set_row_sample.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.oracle_set_grad_class.argtypes = [ctypes.c_int]
+        L.oracle_set_grad_centre.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -165,13 +167,21 @@ class OracleERI:
             raise MemoryError("system too large for the full-tensor oracle")
         return J, K
 
-    def get_gradients(self, P, alpha=1.0, beta=0.0, mode=1):
-        """(dJ, dK), each 3*natoms xyz-major (libint2_direct.cpp:824)."""
+    def get_gradients(self, P, alpha=1.0, beta=0.0, mode=1, class_key=-1, centre=-1):
+        """(dJ, dK), each 3*natoms xyz-major (libint2_direct.cpp:824); class_key
+        (1000 la + 100 lb + 10 lc + ld) restricts the sum to one ERI class,
+        centre (0..2) to the derivative by that centre moved against centre D."""
         P = np.ascontiguousarray(P, dtype=np.float64)
+        lib().oracle_set_grad_class(class_key)
+        lib().oracle_set_grad_centre(centre)
         na = self.structure.natoms
         dJ = np.zeros(3 * na)
         dK = np.zeros(3 * na)
-        rc = lib().oracle_gradients(self.h, _p(P), _p(dJ), _p(dK), alpha, beta, mode)
+        try:
+            rc = lib().oracle_gradients(self.h, _p(P), _p(dJ), _p(dK), alpha, beta, mode)
+        finally:
+            lib().oracle_set_grad_class(-1)
+            lib().oracle_set_grad_centre(-1)
         if rc != 0:
             raise RuntimeError("Density matrix contains NaN/Inf values.")
         return dJ, dK

@@ -372,12 +372,23 @@ struct Engine {
     std::vector<std::vector<int>> bycls(ncls);
     for (int i = 0; i < npair; ++i) bycls[hdr[i].cls].push_back(i);
     std::vector<DevBuf<int>> plist(ncls);
+    // f-shell classes stage their (ab|ab) blocks: batches of pairs that fit
+    // a staging buffer of at most kSchwarzStage doubles
+    constexpr size_t kSchwarzStage = (size_t)1 << 24;
+    DevBuf<double> stage;
     for (int c = 0; c < ncls; ++c) {
       if (bycls[c].empty()) continue;
       plist[c].upload(bycls[c], stream);
-      SchwarzLaunch a{plist[c].p, (int)bycls[c].size(), pairs_d.p, ppos_d.p, pk_d.p, &ra, q2.p};
-      launch_schwarz(c, basis.pure, a, stream);
-      HIP_CHECK(hipGetLastError());
+      const int n = (int)bycls[c].size();
+      const size_t per = (size_t)schwarz_block(c);
+      const int batch = per ? (int)std::max<size_t>(1, std::min<size_t>((size_t)n, kSchwarzStage / per)) : n;
+      if (per) stage.alloc(per * (size_t)batch);
+      for (int i0 = 0; i0 < n; i0 += batch) {
+        SchwarzLaunch a{plist[c].p + i0, std::min(batch, n - i0), pairs_d.p, ppos_d.p, pk_d.p, &ra, q2.p,
+                        per ? stage.p : nullptr};
+        launch_schwarz(c, basis.pure, a, stream);
+        HIP_CHECK(hipGetLastError());
+      }
     }
     std::vector<double> q2h(npair);
     HIP_CHECK(hipMemcpyAsync(q2h.data(), q2.p, sizeof(double) * npair, hipMemcpyDeviceToHost, stream));
@@ -533,9 +544,10 @@ struct Engine {
     if (device < 0 || device >= ndev) throw std::invalid_argument("device_id out of range");
     HIP_CHECK(hipSetDevice(device));
     HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    // QFE_SEGMENT_DIGEST=0 selects the per-task atomic digestion for every class
+    // QFE_SEGMENT_DIGEST=0 keeps only the exchange rows in LDS (the density
+    // rows are read from L2), the variant used when both do not fit
     if (const char* e = std::getenv("QFE_SEGMENT_DIGEST"))
-      if (e[0] == '0') lds_max = 0;
+      if (e[0] == '0') lds_max = -lds_max;
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     setup_rys();

@@ -1,7 +1,7 @@
 // Per-class ERI kernels (templates), instantiated one class per translation
 // unit by the build (gen/cls_*.hip) to keep compile times bounded.
-//   k_eri_jk   Rys ERIs of one class (one bra chunk) + J/K digestion
-//              (libint2_direct.cpp:414-567)
+//   k_eri_jk_seg  Rys ERIs of one class chunk for the tasks of one bra pair
+//                 + J/K digestion (libint2_direct.cpp:414-567)
 //   k_schwarz  sum of squares of (ab|ab) for Q_ab = sqrt ||(ab|ab)||_F
 //              (schwarz.cpp:26-78)
 #pragma once
@@ -25,14 +25,52 @@ inline void hsfor(F&& f) {
   hsfor_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-constexpr int kAccMax = 48;  // register budget (doubles) for one thread's accumulators
+// LDS rows a segment kernel holds for bra functions [q0, q1): the a functions
+// touched, then the b functions (all of them once the range spans two a's)
 template <class C>
-constexpr int chunk_size() {
-  int ch = kAccMax / C::NKETC;
-  if (ch < 1) ch = 1;
-  if (ch > C::NBRA) ch = C::NBRA;
-  return ch;
+constexpr int seg_rows(int q0, int q1) {
+  const int nma = (q1 - 1) / C::NSB - q0 / C::NSB + 1;
+  return nma + (nma > 1 ? C::NSB : q1 - q0);
 }
+constexpr int kSegRowsMax = 6;  // K and P rows of nbf doubles: 6 x 16 B x nbf <= 160 KB up to nbf = 1706
+template <class C>
+constexpr int seg_bch(int hi) {
+  for (int b = hi; b > 1; --b) {
+    int worst = 0;
+    for (int q0 = 0; q0 < C::NBRA; q0 += b) {
+      const int r = seg_rows<C>(q0, q0 + b < C::NBRA ? q0 + b : C::NBRA);
+      worst = r > worst ? r : worst;
+    }
+    if (worst <= kSegRowsMax) return b;
+  }
+  return 1;
+}
+
+// Accumulators per thread: bra functions x ket cartesians of one chunk.  48
+// for classes up to d shells (their whole ket block fits); classes with an f
+// shell hold fewer and are also chunked over the ket cartesians, so that no
+// kernel instance spills registers.
+constexpr int kAccMax = 48;
+constexpr int kAccMaxF = 20;
+template <class C>
+struct JKChunks {
+  static constexpr bool F = C::LA > 2 || C::LB > 2 || C::LC > 2 || C::LD > 2;
+  static constexpr int ACC = F ? kAccMaxF : kAccMax;
+  static constexpr int KCH = C::NKETC <= ACC ? C::NKETC : ACC;  // ket cartesians per chunk
+  static constexpr int NKCH = (C::NKETC + KCH - 1) / KCH;
+  // bra functions per chunk: as many as the accumulators allow, then fewer
+  // until no chunk's segment kernel needs more than kSegRowsMax LDS rows
+  static constexpr int BCH_ = ACC / KCH < 1 ? 1 : (ACC / KCH > C::NBRA ? C::NBRA : ACC / KCH);
+  static constexpr int BCH = seg_bch<C>(BCH_);
+  static constexpr int NBCH = (C::NBRA + BCH - 1) / BCH;
+  static constexpr int NCH = NBCH * NKCH;  // chunk c: bra chunk c / NKCH, ket chunk c % NKCH
+  template <int CI>
+  struct At {
+    static constexpr int b = CI / NKCH, k = CI % NKCH;
+    static constexpr int Q0 = b * BCH, Q1 = (Q0 + BCH < C::NBRA) ? Q0 + BCH : C::NBRA;
+    static constexpr int K0 = k * KCH, K1 = (K0 + KCH < C::NKETC) ? K0 + KCH : C::NKETC;
+  };
+};
 
 // sum over the 64 lanes of a full wave (result valid on every lane)
 QFE_DI double wave_sum(double v) {
@@ -43,100 +81,6 @@ QFE_DI double wave_sum(double v) {
   v += __shfl_xor(v, 2);
   v += __shfl_xor(v, 1);
   return v;
-}
-
-// ------------------------------------------------------------ J/K kernel
-template <class C, int Q0, int Q1, typename R>
-__global__ __launch_bounds__(256) void k_eri_jk(const int2* __restrict__ tasks, long long ntask,
-                                                const PairHdr* __restrict__ pairs,
-                                                const typename Vec4<R>::type* __restrict__ ppos,
-                                                const R* __restrict__ pk, RysArgsT<R> ra,
-                                                const double* __restrict__ P, double* __restrict__ J,
-                                                double* __restrict__ K, int nbf, double kscale) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntask) return;
-  const int2 tk = tasks[t];
-  const PairHdr bra = pairs[tk.x];
-  const PairHdr ket = pairs[tk.y];
-  constexpr int NQ = Q1 - Q0;
-  R acc[NQ][C::NKETC];
-  contract_chunk<C, Q0, Q1, R>(bra, ket, ppos, pk, ra, acc);
-
-  const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
-                     (tk.x == tk.y ? 1.0 : 2.0);
-  // is the bra pair the same on every active lane of this wave?
-  const int lane = (int)(threadIdx.x & 63);
-  const unsigned long long active = __ballot(1);
-  const int bra0 = __shfl(tk.x, __ffsll((long long)active) - 1);
-  const bool uniform_bra = (active == ~0ull) && (__ballot(tk.x != bra0) == 0ull);
-  const int i0 = bra.bfa, j0 = bra.bfb, k0 = ket.bfa, l0 = ket.bfb;
-  const size_t n = (size_t)nbf;
-  constexpr int MA0 = Q0 / C::NSB, MA1 = (Q1 - 1) / C::NSB;
-  constexpr int NMA = MA1 - MA0 + 1;
-  // density blocks needed by the digestion
-  double Pkl[C::NKETS];
-  sfor<C::NSC>([&](auto c) { sfor<C::NSD>([&](auto d) { Pkl[c * C::NSD + d] = P[(k0 + c) * n + l0 + d]; }); });
-  double jkl[C::NKETS];
-  sfor<C::NKETS>([&](auto x) { jkl[x] = 0.0; });
-  double kik[NMA][C::NSC], kil[NMA][C::NSD], kjk[C::NSB][C::NSC], kjl[C::NSB][C::NSD];
-  sfor<NMA>([&](auto a) {
-    sfor<C::NSC>([&](auto c) { kik[a][c] = 0.0; });
-    sfor<C::NSD>([&](auto d) { kil[a][d] = 0.0; });
-  });
-  sfor<C::NSB>([&](auto b) {
-    sfor<C::NSC>([&](auto c) { kjk[b][c] = 0.0; });
-    sfor<C::NSD>([&](auto d) { kjl[b][d] = 0.0; });
-  });
-  const double kq = 0.25 * kscale;
-  sfor<NQ>([&](auto qq) {
-    constexpr int qi = Q0 + qq;
-    constexpr int ma = qi / C::NSB, mb = qi % C::NSB;
-    constexpr int ia = ma - MA0;
-    const size_t ri = (size_t)(i0 + ma) * n, rj = (size_t)(j0 + mb) * n;
-    double v[C::NKETS];
-    ket_to_sph<C, R>(acc[qq], v);
-    const double Pij = P[ri + j0 + mb];
-    double jij = 0.0;
-    sfor<C::NSC>([&](auto c) {
-      const double Pik = P[ri + k0 + c], Pjk = P[rj + k0 + c];
-      sfor<C::NSD>([&](auto d) {
-        const double x = v[c * C::NSD + d] * deg;
-        const double Pjl = P[rj + l0 + d], Pil = P[ri + l0 + d];
-        jij = fma(Pkl[c * C::NSD + d], x, jij);
-        jkl[c * C::NSD + d] = fma(Pij, x, jkl[c * C::NSD + d]);
-        kik[ia][c] = fma(Pjl, x, kik[ia][c]);
-        kjk[mb][c] = fma(Pil, x, kjk[mb][c]);
-        kil[ia][d] = fma(Pjk, x, kil[ia][d]);
-        kjl[mb][d] = fma(Pik, x, kjl[mb][d]);
-      });
-    });
-    if (uniform_bra) {
-      // every lane holds the same (i, j): one atomic per wave
-      const double tot = wave_sum(jij);
-      if (lane == 0) unsafeAtomicAdd(&J[ri + j0 + mb], tot);
-    } else {
-      unsafeAtomicAdd(&J[ri + j0 + mb], jij);
-    }
-  });
-  sfor<C::NSC>([&](auto c) {
-    sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&J[(k0 + c) * n + l0 + d], jkl[c * C::NSD + d]); });
-  });
-  if (kscale != 0.0) {
-    sfor<NMA>([&](auto a) {
-      const size_t ri = (size_t)(i0 + MA0 + a) * n;
-      sfor<C::NSC>([&](auto c) { unsafeAtomicAdd(&K[ri + k0 + c], kq * kik[a][c]); });
-      sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&K[ri + l0 + d], kq * kil[a][d]); });
-    });
-    // bra-b rows touched by this chunk
-    constexpr int MB_LO = (NMA > 1) ? 0 : (Q0 % C::NSB);
-    constexpr int MB_HI = (NMA > 1) ? C::NSB - 1 : ((Q1 - 1) % C::NSB);
-    sfor<MB_HI - MB_LO + 1>([&](auto bb) {
-      constexpr int b = MB_LO + bb;
-      const size_t rj = (size_t)(j0 + b) * n;
-      sfor<C::NSC>([&](auto c) { unsafeAtomicAdd(&K[rj + k0 + c], kq * kjk[b][c]); });
-      sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&K[rj + l0 + d], kq * kjl[b][d]); });
-    });
-  }
 }
 
 // ------------------------------------------------------------ segment J/K kernel
@@ -158,7 +102,7 @@ struct SegRows {
   static constexpr int NR = NMA + NMB;  // LDS rows: a functions, then b functions
 };
 
-template <class C, int Q0, int Q1, typename R, bool PROW>
+template <class C, int Q0, int Q1, typename R, bool PROW, int K0, int K1>
 __global__ __launch_bounds__(256) void k_eri_jk_seg(const int2* __restrict__ segs,
                                                     const long long* __restrict__ offs,
                                                     const int2* __restrict__ tasks,
@@ -205,8 +149,8 @@ __global__ __launch_bounds__(256) void k_eri_jk_seg(const int2* __restrict__ seg
   for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int2 tk = tasks[t];
     const PairHdr ket = pairs[tk.y];
-    R acc[NQ][C::NKETC];
-    contract_chunk<C, Q0, Q1, R>(bra, ket, ppos, pk, ra, acc);
+    R acc[NQ][K1 - K0];
+    contract_chunk<C, Q0, Q1, R, K0, K1>(bra, ket, ppos, pk, ra, acc);
     const double deg = dbra * (ket.sa == ket.sb ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
     const int k0 = ket.bfa, l0 = ket.bfb;
     double Pkl[C::NKETS], jkl[C::NKETS];
@@ -229,7 +173,7 @@ __global__ __launch_bounds__(256) void k_eri_jk_seg(const int2* __restrict__ seg
       constexpr int qi = Q0 + qq;
       constexpr int ia = qi / C::NSB - S::MA0, ib = qi % C::NSB - S::MB_LO;
       double v[C::NKETS];
-      ket_to_sph<C, R>(acc[qq], v);
+      ket_to_sph<C, R, K0, K1>(acc[qq], v);
       sfor<C::NSC>([&](auto c) {
         const double Pac = prow(ia, k0 + c), Pbc = prow(S::NMA + ib, k0 + c);
         sfor<C::NSD>([&](auto d) {
@@ -303,22 +247,66 @@ __global__ __launch_bounds__(256) void k_schwarz(const int* __restrict__ plist, 
   unsafeAtomicAdd(&q2[p], s);
 }
 
+// Schwarz of classes with f shells, in two steps: each chunk kernel writes its
+// share (bra functions x ket cartesians) of the (ab|ab) block to blk, then the
+// fold transforms every bra function's ket row to output functions and adds
+// its sum of squares to q2 (the square needs the whole row, so it cannot be
+// taken per chunk).
+template <class C, int Q0, int Q1, int K0, int K1>
+__global__ __launch_bounds__(256) void k_schwarz_part(const int* __restrict__ plist, int npair,
+                                                      const PairHdr* __restrict__ pairs,
+                                                      const double4* __restrict__ ppos,
+                                                      const double* __restrict__ pk, RysArgs ra,
+                                                      double* __restrict__ blk) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npair) return;
+  const PairHdr pr = pairs[plist[t]];
+  double acc[Q1 - Q0][K1 - K0];
+  contract_chunk<C, Q0, Q1, double, K0, K1>(pr, pr, ppos, pk, ra, acc);
+  sfor<Q1 - Q0>([&](auto qq) {
+    double* row = blk + ((size_t)t * C::NBRA + Q0 + qq) * C::NKETC + K0;
+    sfor<K1 - K0>([&](auto kk) { row[kk] = acc[qq][kk]; });
+  });
+}
+
+template <class C>
+__global__ __launch_bounds__(256) void k_schwarz_fold(const int* __restrict__ plist, int npair,
+                                                      const double* __restrict__ blk, double* __restrict__ q2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)npair * C::NBRA) return;
+  const double* row = blk + (size_t)i * C::NKETC;
+  double s = 0.0;
+  sfor<C::NSC>([&](auto mc) {
+    sfor<C::NSD>([&](auto md) {
+      double v = 0.0;
+      sfor<C::NKETC>([&](auto k) {
+        constexpr double c = sph_coef(C::LC, C::PURE, mc, k / C::NCD) * sph_coef(C::LD, C::PURE, md, k % C::NCD);
+        if constexpr (c != 0.0) v = fma(c, row[k], v);
+      });
+      s = fma(v, v, s);
+    });
+  });
+  unsafeAtomicAdd(&q2[plist[i / C::NBRA]], s);
+}
+
 // ------------------------------------------------------------ launchers
 template <int LA, int LB, int LC, int LD, bool PURE>
 struct ClassOps {
   using C = Cls<LA, LB, LC, LD, PURE>;
-  static constexpr int CH = chunk_size<C>();
-  static constexpr int NCH = (C::NBRA + CH - 1) / CH;
+  using K = JKChunks<C>;
+  static constexpr int NCH = K::NCH;
+  // doubles of the Schwarz staging block per pair (classes with f shells)
+  static constexpr int SCHWARZ_BLOCK = K::F ? C::NBRA * C::NKETC : 0;
 
-  // LDS bytes of the segment kernel for chunk c (K rows, and P rows with PROW)
+  // LDS bytes of the segment kernel (K rows, and P rows with prow)
   template <int Q0, int Q1>
   static size_t seg_bytes(int nbf, bool prow) {
     return (size_t)SegRows<C, Q0, Q1>::NR * nbf * sizeof(double) * (prow ? 2 : 1);
   }
 
-  template <int Q0, int Q1, typename R, bool PROW>
+  template <int Q0, int Q1, int K0, int K1, typename R, bool PROW>
   static bool launch_seg(const JKLaunch& a, hipStream_t s, size_t bytes) {
-    auto kern = k_eri_jk_seg<C, Q0, Q1, R, PROW>;
+    auto kern = k_eri_jk_seg<C, Q0, Q1, R, PROW, K0, K1>;
     if (bytes > 65536 &&
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -333,48 +321,44 @@ struct ClassOps {
     return true;
   }
 
-  // segment kernel when its K and P rows fit in LDS (else the per-task kernel)
-  template <int Q0, int Q1, typename R>
-  static bool try_seg(const JKLaunch& a, hipStream_t s) {
-    if (a.nseg <= 0 || a.lds_max <= 0 || a.segs == nullptr) return false;
-    const size_t bytes = seg_bytes<Q0, Q1>(a.nbf, true);
-    return bytes <= (size_t)a.lds_max && launch_seg<Q0, Q1, R, true>(a, s, bytes);
+  // segment kernel with K and P rows in LDS when they fit, else K rows only
+  // (P from L2); a negative lds_max forces the K-only variant
+  template <int Q0, int Q1, int K0, int K1, typename R, bool BOTH>
+  static void seg(const JKLaunch& a, hipStream_t s) {
+    static_assert(SegRows<C, Q0, Q1>::NR <= kSegRowsMax, "chunk needs more LDS rows than planned");
+    if (a.nseg <= 0 || a.segs == nullptr) throw std::runtime_error("segment list missing");
+    const size_t lim = (size_t)(a.lds_max < 0 ? -a.lds_max : a.lds_max);
+    const size_t with_p = seg_bytes<Q0, Q1>(a.nbf, true), k_only = seg_bytes<Q0, Q1>(a.nbf, false);
+    if (a.lds_max > 0 && with_p <= lim && launch_seg<Q0, Q1, K0, K1, R, true>(a, s, with_p)) return;
+    if constexpr (BOTH)
+      if (k_only <= lim && launch_seg<Q0, Q1, K0, K1, R, false>(a, s, k_only)) return;
+    throw std::runtime_error("the exchange rows of one bra pair exceed the LDS of a block (nbf too large)");
   }
 
-  // Bra-function chunks [C0, C1) of the class; the build may split the chunks
-  // of a heavy class over several translation units (LAST marks the range
-  // that must end at NCH, so a mismatch with the build fails to compile).
   template <int C0, int C1, bool LAST>
   static void jk_range(const JKLaunch& a, hipStream_t s) {
     static_assert(C0 < C1 && C1 <= NCH && (!LAST || C1 == NCH), "chunk range does not match the class");
+    static_assert(!K::F, "classes with f shells use jk_seg_only_range");
     if (a.ntask <= 0) return;
-    const int block = 256;
-    const long long grid = (a.ntask + block - 1) / block;
     hsfor<C1 - C0>([&](auto cc) {
-      constexpr int q0 = (C0 + cc) * CH;
-      constexpr int q1 = (q0 + CH < C::NBRA) ? q0 + CH : C::NBRA;
-      if (a.fp32 ? try_seg<q0, q1, float>(a, s) : try_seg<q0, q1, double>(a, s)) return;
+      using A = typename K::template At<C0 + cc>;
       if (a.fp32)
-        hipLaunchKernelGGL((k_eri_jk<C, q0, q1, float>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
-                           a.ntask, a.pairs, a.ppos32, a.pk32, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale);
+        seg<A::Q0, A::Q1, A::K0, A::K1, float, true>(a, s);
       else
-        hipLaunchKernelGGL((k_eri_jk<C, q0, q1, double>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
-                           a.ntask, a.pairs, a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale);
+        seg<A::Q0, A::Q1, A::K0, A::K1, double, true>(a, s);
     });
   }
 
   // FP64 segment kernels only (FP32 tasks are evaluated in FP64): the variant
-  // compiled for classes with f shells, where every chunk holds one bra
-  // function and the per-task kernel would double the compile time
+  // compiled for classes with f shells, where the per-task kernel would double
+  // the compile time
   template <int C0, int C1, bool LAST>
   static void jk_seg_only_range(const JKLaunch& a, hipStream_t s) {
     static_assert(C0 < C1 && C1 <= NCH && (!LAST || C1 == NCH), "chunk range does not match the class");
     if (a.ntask <= 0) return;
     hsfor<C1 - C0>([&](auto cc) {
-      constexpr int q0 = (C0 + cc) * CH;
-      constexpr int q1 = (q0 + CH < C::NBRA) ? q0 + CH : C::NBRA;
-      if (!try_seg<q0, q1, double>(a, s))
-        throw std::runtime_error("exchange rows of this f-shell class exceed the LDS of one block");
+      using A = typename K::template At<C0 + cc>;
+      seg<A::Q0, A::Q1, A::K0, A::K1, double, false>(a, s);
     });
   }
 
@@ -382,14 +366,28 @@ struct ClassOps {
   static void schwarz_range(const SchwarzLaunch& a, hipStream_t s) {
     static_assert(C0 < C1 && C1 <= NCH && (!LAST || C1 == NCH), "chunk range does not match the class");
     if (a.npair <= 0) return;
+    if (K::F && a.blk == nullptr) throw std::runtime_error("Schwarz staging block missing for an f-shell class");
     const int block = 256;
     const int grid = (a.npair + block - 1) / block;
     hsfor<C1 - C0>([&](auto cc) {
-      constexpr int q0 = (C0 + cc) * CH;
-      constexpr int q1 = (q0 + CH < C::NBRA) ? q0 + CH : C::NBRA;
-      hipLaunchKernelGGL((k_schwarz<C, q0, q1>), dim3(grid), dim3(block), 0, s, a.plist, a.npair,
-                         a.pairs, a.ppos, a.pk, *a.ra, a.q2);
+      using A = typename K::template At<C0 + cc>;
+      if constexpr (K::F)
+        hipLaunchKernelGGL((k_schwarz_part<C, A::Q0, A::Q1, A::K0, A::K1>), dim3(grid), dim3(block), 0, s, a.plist,
+                           a.npair, a.pairs, a.ppos, a.pk, *a.ra, a.blk);
+      else
+        hipLaunchKernelGGL((k_schwarz<C, A::Q0, A::Q1>), dim3(grid), dim3(block), 0, s, a.plist, a.npair, a.pairs,
+                           a.ppos, a.pk, *a.ra, a.q2);
     });
+  }
+
+  // after every chunk range: fold the staged blocks (classes with f shells)
+  static void schwarz_fold(const SchwarzLaunch& a, hipStream_t s) {
+    if constexpr (K::F) {
+      if (a.npair <= 0) return;
+      const long long n = (long long)a.npair * C::NBRA;
+      hipLaunchKernelGGL((k_schwarz_fold<C>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.plist, a.npair,
+                         a.blk, a.q2);
+    }
   }
 };
 

@@ -28,7 +28,9 @@
 namespace qfe {
 
 #define QFE_HD __host__ __device__
-#define QFE_DI __device__ __forceinline__
+// device functions are also host-callable, so the kernel arithmetic can be
+// checked on the CPU (the hardware estimates below fall back to exact math)
+#define QFE_DI __host__ __device__ __forceinline__
 
 // ---------------------------------------------------------------- indices
 QFE_HD constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
@@ -103,24 +105,40 @@ constexpr int BOYS_W = 24;
 // 1/x and 1/sqrt(x) for positive normal x: hardware estimate plus two
 // Newton steps (quadratic convergence to full double precision).
 QFE_DI double frcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(x);
   r = fma(fma(-x, r, 1.0), r, r);
   return fma(fma(-x, r, 1.0), r, r);
+#else
+  return 1.0 / x;
+#endif
 }
 QFE_DI double frsq(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rsq(x);
   const double hx = 0.5 * x;
   r = r * fma(-hx * r, r, 1.5);
   return r * fma(-hx * r, r, 1.5);
+#else
+  return 1.0 / sqrt(x);
+#endif
 }
 // single precision: one Newton step on the hardware estimate
 QFE_DI float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
   const float r = __builtin_amdgcn_rcpf(x);
   return fmaf(fmaf(-x, r, 1.0f), r, r);
+#else
+  return 1.0f / x;
+#endif
 }
 QFE_DI float frsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
   const float r = __builtin_amdgcn_rsqf(x);
   return r * fmaf(-0.5f * x * r, r, 1.5f);
+#else
+  return 1.0f / sqrtf(x);
+#endif
 }
 QFE_DI double fexp(double x) { return exp(x); }
 QFE_DI float fexp(float x) { return expf(x); }
@@ -309,15 +327,16 @@ QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, 
   rys_1d_x<C::EA, C::FC, C::LA, C::LB, C::LC, C::LD, R>(s, C00, C00p, B00, B10, B01, AB, CD, out);
 }
 
-// Contract all primitive quartets of (bra|ket) into acc[q][kc] for the bra
-// function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and all ket
-// cartesian pairs kc (row-major over (c, d)), in precision R.
-template <class C, int Q0, int Q1, typename R>
+// Contract all primitive quartets of (bra|ket) into acc[q][k] for the bra
+// function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and the ket
+// cartesian pairs K0 + k, k < K1 - K0 (row-major over (c, d)), in precision R.
+template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC>
 QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
                            const typename Vec4<R>::type* __restrict__ ppos, const R* __restrict__ pk,
-                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][C::NKETC]) {
+                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][K1 - K0]) {
+  constexpr int NK = K1 - K0;
   const R TWO_PI_52 = R(34.98683665524972);  // 2 pi^(5/2)
-  sfor<Q1 - Q0>([&](auto q) { sfor<C::NKETC>([&](auto k) { acc[q][k] = R(0); }); });
+  sfor<Q1 - Q0>([&](auto q) { sfor<NK>([&](auto k) { acc[q][k] = R(0); }); });
   const R ABx = R(bra.A[0] - bra.B[0]), ABy = R(bra.A[1] - bra.B[1]), ABz = R(bra.A[2] - bra.B[2]);
   const R CDx = R(ket.A[0] - ket.B[0]), CDy = R(ket.A[1] - ket.B[1]), CDz = R(ket.A[2] - ket.B[2]);
   const R Ax = R(bra.A[0]), Ay = R(bra.A[1]), Az = R(bra.A[2]);
@@ -360,9 +379,10 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
         sfor<Q1 - Q0>([&](auto qq) {
           constexpr int qi = Q0 + qq;
           constexpr int ma = qi / C::NSB, mb = qi % C::NSB;
-          sfor<C::NCC>([&](auto cc) {
+          sfor<NK>([&](auto kk) {
+            constexpr int cc = (K0 + kk) / C::NCD, dc = (K0 + kk) % C::NCD;
             constexpr int cx = cart_x(C::LC, cc), cy = cart_y(C::LC, cc), cz = cart_z(C::LC, cc);
-            sfor<C::NCD>([&](auto dc) {
+            {
               constexpr int dx = cart_x(C::LD, dc), dy = cart_y(C::LD, dc), dz = cart_z(C::LD, dc);
               R s = R(0);
               sfor<C::NCA>([&](auto ac) {
@@ -382,8 +402,8 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
                   });
                 }
               });
-              acc[qq][cc * C::NCD + dc] += s;
-            });
+              acc[qq][kk] += s;
+            }
           });
         });
       }
@@ -391,20 +411,18 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
   }
 }
 
-// ket cartesian -> output functions for one bra function pair
-template <class C, typename R>
-QFE_DI void ket_to_sph(const R (&in)[C::NKETC], double (&out)[C::NKETS]) {
+// ket cartesian -> output functions for one bra function pair; with a ket
+// cartesian range [K0, K1) the result is that range's share of every output
+// function (the transform is linear, so the shares of all ranges add up)
+template <class C, typename R, int K0 = 0, int K1 = C::NKETC>
+QFE_DI void ket_to_sph(const R (&in)[K1 - K0], double (&out)[C::NKETS]) {
   sfor<C::NSC>([&](auto mc) {
     sfor<C::NSD>([&](auto md) {
       double s = 0.0;
-      sfor<C::NCC>([&](auto cc) {
-        constexpr double c1 = sph_coef(C::LC, C::PURE, mc, cc);
-        if constexpr (c1 != 0.0) {
-          sfor<C::NCD>([&](auto dc) {
-            constexpr double c2 = sph_coef(C::LD, C::PURE, md, dc);
-            if constexpr (c2 != 0.0) s = fma(c1 * c2, (double)in[cc * C::NCD + dc], s);
-          });
-        }
+      sfor<K1 - K0>([&](auto kk) {
+        constexpr int cc = (K0 + kk) / C::NCD, dc = (K0 + kk) % C::NCD;
+        constexpr double c = sph_coef(C::LC, C::PURE, mc, cc) * sph_coef(C::LD, C::PURE, md, dc);
+        if constexpr (c != 0.0) s = fma(c, (double)in[kk], s);
       });
       out[mc * C::NSD + md] = s;
     });

@@ -27,14 +27,20 @@ struct GradTraits {
   static constexpr int NBRAC = C::NCA * C::NCB;  // bra cartesian pairs
 };
 
-constexpr int kGradAccMax = 24;  // weights held per thread: 2 * chunk * ket cartesians
+// Weights held per thread (bra cartesians x ket cartesians of one chunk): 12
+// keeps every class up to (dd|dd) free of register spills; classes with more
+// ket cartesians than that are also chunked over the ket.
+constexpr int kGradWeights = 12;
 template <class C>
-constexpr int grad_chunk() {
-  int ch = kGradAccMax / (2 * C::NKETC);
-  if (ch < 1) ch = 1;
-  if (ch > C::NCA * C::NCB) ch = C::NCA * C::NCB;
-  return ch;
-}
+struct GradChunks {
+  static constexpr int NBC = C::NCA * C::NCB;
+  static constexpr int KCH = C::NKETC <= kGradWeights ? C::NKETC : kGradWeights;  // ket cartesians per chunk
+  static constexpr int NKCH = (C::NKETC + KCH - 1) / KCH;
+  static constexpr int BCH_ = kGradWeights / KCH;
+  static constexpr int BCH = BCH_ < 1 ? 1 : (BCH_ > NBC ? NBC : BCH_);  // bra cartesians per chunk
+  static constexpr int NBCH = (NBC + BCH - 1) / BCH;
+  static constexpr int NCH = NBCH * NKCH;  // chunk c: bra chunk c / NKCH, ket chunk c % NKCH
+};
 
 // density block (sa, sb) back-transformed to cartesians: Pc[x][y] =
 // sum_{m,n} S_la[m][x] S_lb[n][y] P[bf_a + m][bf_b + n]
@@ -61,135 +67,152 @@ QFE_DI void cart_density(const double* __restrict__ P, size_t n, int bx, int by,
   });
 }
 
-template <class C, int Q0, int Q1, bool LDS>
+// Derivatives of one canonical task (bra pair tk.x, ket pair tk.y) by its
+// centres A, B and C, contracted with the J and K weights: gj[centre][xyz],
+// gk[centre][xyz] (centre D follows by translation invariance).  Host-callable
+// so the arithmetic can be checked on the CPU.
+template <class C, int Q0, int Q1, int K0, int K1>
+QFE_DI void grad_task(int2 tk, const PairHdr* __restrict__ pairs, const double4* __restrict__ ppos,
+                      const double* __restrict__ pk, const double* __restrict__ pa, const RysArgs& ra,
+                      const double* __restrict__ P, int nbf, double kscale, double (&gj)[3][3],
+                      double (&gk)[3][3]) {
+  using G = GradTraits<C>;
+  const PairHdr bra = pairs[tk.x];
+  const PairHdr ket = pairs[tk.y];
+  const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
+                     (tk.x == tk.y ? 1.0 : 2.0);
+  const size_t n = (size_t)nbf;
+  // cartesian density blocks
+  double Pab[C::NCA][C::NCB], Pcd[C::NCC][C::NCD], Pac[C::NCA][C::NCC], Pbd[C::NCB][C::NCD],
+      Pad[C::NCA][C::NCD], Pbc[C::NCB][C::NCC];
+  cart_density<C::LA, C::LB, C::PURE>(P, n, bra.bfa, bra.bfb, Pab);
+  cart_density<C::LC, C::LD, C::PURE>(P, n, ket.bfa, ket.bfb, Pcd);
+  cart_density<C::LA, C::LC, C::PURE>(P, n, bra.bfa, ket.bfa, Pac);
+  cart_density<C::LB, C::LD, C::PURE>(P, n, bra.bfb, ket.bfb, Pbd);
+  cart_density<C::LA, C::LD, C::PURE>(P, n, bra.bfa, ket.bfb, Pad);
+  cart_density<C::LB, C::LC, C::PURE>(P, n, bra.bfb, ket.bfa, Pbc);
+  constexpr int NQ = Q1 - Q0, NK = K1 - K0;
+  double WJ[NQ][NK], WK[NQ][NK];
+  const double wj = 0.5 * deg, wk = -0.125 * kscale * deg;
+  sfor<NQ>([&](auto qq) {
+    constexpr int ac = (Q0 + qq) / C::NCB, bc = (Q0 + qq) % C::NCB;
+    sfor<NK>([&](auto kk) {
+      constexpr int cc = (K0 + kk) / C::NCD, dc = (K0 + kk) % C::NCD;
+      WJ[qq][kk] = wj * Pab[ac][bc] * Pcd[cc][dc];
+      WK[qq][kk] = wk * (Pac[ac][cc] * Pbd[bc][dc] + Pad[ac][dc] * Pbc[bc][cc]);
+    });
+  });
+  sfor<3>([&](auto a) { sfor<3>([&](auto b) { gj[a][b] = 0.0; gk[a][b] = 0.0; }); });
+  constexpr double TWO_PI_52 = 34.98683665524972;
+  const double ABx = bra.A[0] - bra.B[0], ABy = bra.A[1] - bra.B[1], ABz = bra.A[2] - bra.B[2];
+  const double CDx = ket.A[0] - ket.B[0], CDy = ket.A[1] - ket.B[1], CDz = ket.A[2] - ket.B[2];
+  for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
+    const double4 Q = ppos[ik];
+    const double Kk = pk[ik];
+    const double q = Q.x;
+    const double tg = 2.0 * pa[ik];  // 2 gamma (exponent on centre C)
+    const double QCx = Q.y - ket.A[0], QCy = Q.z - ket.A[1], QCz = Q.w - ket.A[2];
+    const double h2q = 0.5 * frcp(q);
+    for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
+      const double4 Pp = ppos[ib];
+      const double Kb = pk[ib];
+      const double p = Pp.x;
+      const double ta = 2.0 * pa[ib], tb = 2.0 * (p - pa[ib]);  // 2 alpha, 2 beta
+      const double PAx = Pp.y - bra.A[0], PAy = Pp.z - bra.A[1], PAz = Pp.w - bra.A[2];
+      const double PQx = Pp.y - Q.y, PQy = Pp.z - Q.z, PQz = Pp.w - Q.w;
+      const double rs = frsq(p + q);
+      const double rpq = rs * rs;
+      const double T = p * q * rpq * (PQx * PQx + PQy * PQy + PQz * PQz);
+      const double pref = TWO_PI_52 * Kb * Kk * rs;
+      double u[G::NR1], w[G::NR1];
+      rys_roots<G::NR1>(T, ra, u, w);
+      const double qr = q * rpq, pr = p * rpq;
+      const double h2p = 0.5 * frcp(p);
+#pragma unroll 1
+      for (int r = 0; r < G::NR1; ++r) {
+        const double ur = u[r];
+        const double B00 = 0.5 * rpq * ur;
+        const double B10 = h2p - h2p * qr * ur;
+        const double B01 = h2q - h2q * pr * ur;
+        const double xa = qr * ur, xc = pr * ur;
+        OneDX<C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double> I[3];
+        rys_1d_x<G::EA1, G::FC1, C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double>(
+            1.0, PAx - xa * PQx, QCx + xc * PQx, B00, B10, B01, ABx, CDx, I[0]);
+        rys_1d_x<G::EA1, G::FC1, C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double>(
+            1.0, PAy - xa * PQy, QCy + xc * PQy, B00, B10, B01, ABy, CDy, I[1]);
+        rys_1d_x<G::EA1, G::FC1, C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double>(
+            w[r] * pref, PAz - xa * PQz, QCz + xc * PQz, B00, B10, B01, ABz, CDz, I[2]);
+        sfor<NQ>([&](auto qq) {
+          constexpr int ac = (Q0 + qq) / C::NCB, bc = (Q0 + qq) % C::NCB;
+          constexpr int ae[3] = {cart_x(C::LA, ac), cart_y(C::LA, ac), cart_z(C::LA, ac)};
+          constexpr int be[3] = {cart_x(C::LB, bc), cart_y(C::LB, bc), cart_z(C::LB, bc)};
+          sfor<NK>([&](auto kk) {
+            constexpr int cc = (K0 + kk) / C::NCD, dc = (K0 + kk) % C::NCD;
+            constexpr int ce[3] = {cart_x(C::LC, cc), cart_y(C::LC, cc), cart_z(C::LC, cc)};
+            {
+              constexpr int de[3] = {cart_x(C::LD, dc), cart_y(C::LD, dc), cart_z(C::LD, dc)};
+              const double wJ = WJ[qq][kk], wK = WK[qq][kk];
+              double v[3];
+              sfor<3>([&](auto x) { v[x] = I[x].v[ae[x]][be[x]][ce[x]][de[x]]; });
+              sfor<3>([&](auto x) {
+                constexpr int y = (x + 1) % 3, z = (x + 2) % 3;
+                const double o = v[y] * v[z];
+                const double oj = wJ * o, ok = wK * o;
+                // centre A
+                double dA = ta * I[x].v[ae[x] + 1][be[x]][ce[x]][de[x]];
+                if constexpr (ae[x] > 0) dA -= (double)ae[x] * I[x].v[ae[x] - 1][be[x]][ce[x]][de[x]];
+                // centre B
+                double dB = tb * I[x].v[ae[x]][be[x] + 1][ce[x]][de[x]];
+                if constexpr (be[x] > 0) dB -= (double)be[x] * I[x].v[ae[x]][be[x] - 1][ce[x]][de[x]];
+                // centre C
+                double dC = tg * I[x].v[ae[x]][be[x]][ce[x] + 1][de[x]];
+                if constexpr (ce[x] > 0) dC -= (double)ce[x] * I[x].v[ae[x]][be[x]][ce[x] - 1][de[x]];
+                gj[0][x] = fma(dA, oj, gj[0][x]);
+                gk[0][x] = fma(dA, ok, gk[0][x]);
+                gj[1][x] = fma(dB, oj, gj[1][x]);
+                gk[1][x] = fma(dB, ok, gk[1][x]);
+                gj[2][x] = fma(dC, oj, gj[2][x]);
+                gk[2][x] = fma(dC, ok, gk[2][x]);
+              });
+            }
+          });
+        });
+      }
+    }
+  }
+}
+
+// One thread per canonical task; bra cartesians [Q0, Q1) and ket cartesians
+// [K0, K1) of the class (the chunks keep every instance free of register
+// spills).  With lds the block sums its contributions in LDS ([2][3][natoms])
+// and flushes one atomic per entry; otherwise each thread adds to global.
+template <class C, int Q0, int Q1, int K0, int K1>
 __global__ __launch_bounds__(256) void k_eri_grad(const int2* __restrict__ tasks, long long ntask,
                                                   const PairHdr* __restrict__ pairs,
                                                   const double4* __restrict__ ppos, const double* __restrict__ pk,
                                                   const double* __restrict__ pa, RysArgs ra,
                                                   const double* __restrict__ P, int nbf,
                                                   const int* __restrict__ sh_atom, int natoms, double kscale,
-                                                  double* __restrict__ gJ, double* __restrict__ gK) {
-  using G = GradTraits<C>;
-  extern __shared__ double sg[];  // [2][3][natoms] when LDS
-  if constexpr (LDS) {
+                                                  int lds, double* __restrict__ gJ, double* __restrict__ gK) {
+  extern __shared__ double sg[];
+  if (lds) {
     for (int i = threadIdx.x; i < 6 * natoms; i += blockDim.x) sg[i] = 0.0;
     __syncthreads();
   }
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < ntask) {
     const int2 tk = tasks[t];
-    const PairHdr bra = pairs[tk.x];
-    const PairHdr ket = pairs[tk.y];
-    const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
-                       (tk.x == tk.y ? 1.0 : 2.0);
-    const size_t n = (size_t)nbf;
-    // cartesian density blocks
-    double Pab[C::NCA][C::NCB], Pcd[C::NCC][C::NCD], Pac[C::NCA][C::NCC], Pbd[C::NCB][C::NCD],
-        Pad[C::NCA][C::NCD], Pbc[C::NCB][C::NCC];
-    cart_density<C::LA, C::LB, C::PURE>(P, n, bra.bfa, bra.bfb, Pab);
-    cart_density<C::LC, C::LD, C::PURE>(P, n, ket.bfa, ket.bfb, Pcd);
-    cart_density<C::LA, C::LC, C::PURE>(P, n, bra.bfa, ket.bfa, Pac);
-    cart_density<C::LB, C::LD, C::PURE>(P, n, bra.bfb, ket.bfb, Pbd);
-    cart_density<C::LA, C::LD, C::PURE>(P, n, bra.bfa, ket.bfb, Pad);
-    cart_density<C::LB, C::LC, C::PURE>(P, n, bra.bfb, ket.bfa, Pbc);
-    constexpr int NQ = Q1 - Q0;
-    double WJ[NQ][C::NKETC], WK[NQ][C::NKETC];
-    const double wj = 0.5 * deg, wk = -0.125 * kscale * deg;
-    sfor<NQ>([&](auto qq) {
-      constexpr int ac = (Q0 + qq) / C::NCB, bc = (Q0 + qq) % C::NCB;
-      sfor<C::NCC>([&](auto cc) {
-        sfor<C::NCD>([&](auto dc) {
-          WJ[qq][cc * C::NCD + dc] = wj * Pab[ac][bc] * Pcd[cc][dc];
-          WK[qq][cc * C::NCD + dc] = wk * (Pac[ac][cc] * Pbd[bc][dc] + Pad[ac][dc] * Pbc[bc][cc]);
-        });
-      });
-    });
-    // accumulators [J/K][centre A, B, C][xyz]
     double gj[3][3], gk[3][3];
-    sfor<3>([&](auto a) { sfor<3>([&](auto b) { gj[a][b] = 0.0; gk[a][b] = 0.0; }); });
-    constexpr double TWO_PI_52 = 34.98683665524972;
-    const double ABx = bra.A[0] - bra.B[0], ABy = bra.A[1] - bra.B[1], ABz = bra.A[2] - bra.B[2];
-    const double CDx = ket.A[0] - ket.B[0], CDy = ket.A[1] - ket.B[1], CDz = ket.A[2] - ket.B[2];
-    for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
-      const double4 Q = ppos[ik];
-      const double Kk = pk[ik];
-      const double q = Q.x;
-      const double tg = 2.0 * pa[ik];  // 2 gamma (exponent on centre C)
-      const double QCx = Q.y - ket.A[0], QCy = Q.z - ket.A[1], QCz = Q.w - ket.A[2];
-      const double h2q = 0.5 * frcp(q);
-      for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
-        const double4 Pp = ppos[ib];
-        const double Kb = pk[ib];
-        const double p = Pp.x;
-        const double ta = 2.0 * pa[ib], tb = 2.0 * (p - pa[ib]);  // 2 alpha, 2 beta
-        const double PAx = Pp.y - bra.A[0], PAy = Pp.z - bra.A[1], PAz = Pp.w - bra.A[2];
-        const double PQx = Pp.y - Q.y, PQy = Pp.z - Q.z, PQz = Pp.w - Q.w;
-        const double rs = frsq(p + q);
-        const double rpq = rs * rs;
-        const double T = p * q * rpq * (PQx * PQx + PQy * PQy + PQz * PQz);
-        const double pref = TWO_PI_52 * Kb * Kk * rs;
-        double u[G::NR1], w[G::NR1];
-        rys_roots<G::NR1>(T, ra, u, w);
-        const double qr = q * rpq, pr = p * rpq;
-        const double h2p = 0.5 * frcp(p);
-#pragma unroll 1
-        for (int r = 0; r < G::NR1; ++r) {
-          const double ur = u[r];
-          const double B00 = 0.5 * rpq * ur;
-          const double B10 = h2p - h2p * qr * ur;
-          const double B01 = h2q - h2q * pr * ur;
-          const double xa = qr * ur, xc = pr * ur;
-          OneDX<C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double> I[3];
-          rys_1d_x<G::EA1, G::FC1, C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double>(
-              1.0, PAx - xa * PQx, QCx + xc * PQx, B00, B10, B01, ABx, CDx, I[0]);
-          rys_1d_x<G::EA1, G::FC1, C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double>(
-              1.0, PAy - xa * PQy, QCy + xc * PQy, B00, B10, B01, ABy, CDy, I[1]);
-          rys_1d_x<G::EA1, G::FC1, C::LA + 1, C::LB + 1, C::LC + 1, C::LD, double>(
-              w[r] * pref, PAz - xa * PQz, QCz + xc * PQz, B00, B10, B01, ABz, CDz, I[2]);
-          sfor<NQ>([&](auto qq) {
-            constexpr int ac = (Q0 + qq) / C::NCB, bc = (Q0 + qq) % C::NCB;
-            constexpr int ae[3] = {cart_x(C::LA, ac), cart_y(C::LA, ac), cart_z(C::LA, ac)};
-            constexpr int be[3] = {cart_x(C::LB, bc), cart_y(C::LB, bc), cart_z(C::LB, bc)};
-            sfor<C::NCC>([&](auto cc) {
-              constexpr int ce[3] = {cart_x(C::LC, cc), cart_y(C::LC, cc), cart_z(C::LC, cc)};
-              sfor<C::NCD>([&](auto dc) {
-                constexpr int de[3] = {cart_x(C::LD, dc), cart_y(C::LD, dc), cart_z(C::LD, dc)};
-                const double wJ = WJ[qq][cc * C::NCD + dc], wK = WK[qq][cc * C::NCD + dc];
-                double v[3];
-                sfor<3>([&](auto x) { v[x] = I[x].v[ae[x]][be[x]][ce[x]][de[x]]; });
-                sfor<3>([&](auto x) {
-                  constexpr int y = (x + 1) % 3, z = (x + 2) % 3;
-                  const double o = v[y] * v[z];
-                  const double oj = wJ * o, ok = wK * o;
-                  // centre A
-                  double dA = ta * I[x].v[ae[x] + 1][be[x]][ce[x]][de[x]];
-                  if constexpr (ae[x] > 0) dA -= (double)ae[x] * I[x].v[ae[x] - 1][be[x]][ce[x]][de[x]];
-                  // centre B
-                  double dB = tb * I[x].v[ae[x]][be[x] + 1][ce[x]][de[x]];
-                  if constexpr (be[x] > 0) dB -= (double)be[x] * I[x].v[ae[x]][be[x] - 1][ce[x]][de[x]];
-                  // centre C
-                  double dC = tg * I[x].v[ae[x]][be[x]][ce[x] + 1][de[x]];
-                  if constexpr (ce[x] > 0) dC -= (double)ce[x] * I[x].v[ae[x]][be[x]][ce[x] - 1][de[x]];
-                  gj[0][x] = fma(dA, oj, gj[0][x]);
-                  gk[0][x] = fma(dA, ok, gk[0][x]);
-                  gj[1][x] = fma(dB, oj, gj[1][x]);
-                  gk[1][x] = fma(dB, ok, gk[1][x]);
-                  gj[2][x] = fma(dC, oj, gj[2][x]);
-                  gk[2][x] = fma(dC, ok, gk[2][x]);
-                });
-              });
-            });
-          });
-        }
-      }
-    }
+    grad_task<C, Q0, Q1, K0, K1>(tk, pairs, ppos, pk, pa, ra, P, nbf, kscale, gj, gk);
+    const PairHdr& bra = pairs[tk.x];
+    const PairHdr& ket = pairs[tk.y];
     const int at[4] = {sh_atom[bra.sa], sh_atom[bra.sb], sh_atom[ket.sa], sh_atom[ket.sb]};
     sfor<3>([&](auto x) {
       const double dj[4] = {gj[0][x], gj[1][x], gj[2][x], -(gj[0][x] + gj[1][x] + gj[2][x])};
       const double dk[4] = {gk[0][x], gk[1][x], gk[2][x], -(gk[0][x] + gk[1][x] + gk[2][x])};
       sfor<4>([&](auto c) {
         const int idx = at[c] + x * natoms;
-        if constexpr (LDS) {
+        if (lds) {
           atomicAdd(&sg[idx], dj[c]);
           atomicAdd(&sg[3 * natoms + idx], dk[c]);
         } else {
@@ -199,7 +222,7 @@ __global__ __launch_bounds__(256) void k_eri_grad(const int2* __restrict__ tasks
       });
     });
   }
-  if constexpr (LDS) {
+  if (lds) {
     __syncthreads();
     for (int i = threadIdx.x; i < 3 * natoms; i += blockDim.x) {
       if (sg[i] != 0.0) unsafeAtomicAdd(&gJ[i], sg[i]);
@@ -211,9 +234,8 @@ __global__ __launch_bounds__(256) void k_eri_grad(const int2* __restrict__ tasks
 template <int LA, int LB, int LC, int LD, bool PURE>
 struct GradOps {
   using C = Cls<LA, LB, LC, LD, PURE>;
-  static constexpr int CH = grad_chunk<C>();
-  static constexpr int NBC = C::NCA * C::NCB;
-  static constexpr int NCH = (NBC + CH - 1) / CH;
+  using K = GradChunks<C>;
+  static constexpr int NCH = K::NCH;
 
   template <int C0, int C1, bool LAST>
   static void run_range(const GradLaunch& a, hipStream_t s) {
@@ -221,19 +243,16 @@ struct GradOps {
     if (a.ntask <= 0) return;
     const int block = 256;
     const long long grid = (a.ntask + block - 1) / block;
-    const bool lds = a.natoms * 6 * sizeof(double) <= 64 * 1024;
+    const int lds = a.natoms * 6 * (int)sizeof(double) <= 64 * 1024 ? 1 : 0;
     const size_t shm = lds ? (size_t)a.natoms * 6 * sizeof(double) : 0;
     hsfor<C1 - C0>([&](auto cc) {
-      constexpr int q0 = (C0 + cc) * CH;
-      constexpr int q1 = (q0 + CH < NBC) ? q0 + CH : NBC;
-      if (lds)
-        hipLaunchKernelGGL((k_eri_grad<C, q0, q1, true>), dim3((unsigned)grid), dim3(block), shm, s, a.tasks,
-                           a.ntask, a.pairs, a.ppos, a.pk, a.pa, *a.ra, a.P, a.nbf, a.sh_atom, a.natoms, a.kscale,
-                           a.gJ, a.gK);
-      else
-        hipLaunchKernelGGL((k_eri_grad<C, q0, q1, false>), dim3((unsigned)grid), dim3(block), 0, s, a.tasks,
-                           a.ntask, a.pairs, a.ppos, a.pk, a.pa, *a.ra, a.P, a.nbf, a.sh_atom, a.natoms,
-                           a.kscale, a.gJ, a.gK);
+      constexpr int c = C0 + cc;
+      constexpr int b = c / K::NKCH, k = c % K::NKCH;
+      constexpr int q0 = b * K::BCH, q1 = (q0 + K::BCH < K::NBC) ? q0 + K::BCH : K::NBC;
+      constexpr int k0 = k * K::KCH, k1 = (k0 + K::KCH < C::NKETC) ? k0 + K::KCH : C::NKETC;
+      hipLaunchKernelGGL((k_eri_grad<C, q0, q1, k0, k1>), dim3((unsigned)grid), dim3(block), shm, s, a.tasks,
+                         a.ntask, a.pairs, a.ppos, a.pk, a.pa, *a.ra, a.P, a.nbf, a.sh_atom, a.natoms, a.kscale,
+                         lds, a.gJ, a.gK);
     });
   }
 };

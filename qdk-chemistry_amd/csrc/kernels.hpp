@@ -5,7 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #ifndef QFE_MAX_L
-#define QFE_MAX_L 2  // highest shell angular momentum compiled into the ERI kernels
+#define QFE_MAX_L 3  // highest shell angular momentum compiled into the ERI kernels
 #endif
 
 namespace qfe {
@@ -38,7 +38,7 @@ struct JKLaunch {
   int nseg;
   const long long* offs;
   const int2* tasks_all;
-  int lds_max;        // dynamic LDS bytes a block may request (0: flat kernel only)
+  int lds_max;        // dynamic LDS bytes a block may request (negative: exchange rows only)
 };
 
 struct GradLaunch {
@@ -66,7 +66,12 @@ struct SchwarzLaunch {
   const double* pk;
   const RysArgs* ra;
   double* q2;
+  double* blk;  // staging of the (ab|ab) blocks of f-shell classes: npair x schwarz_block(c) doubles
 };
+
+// doubles of the Schwarz staging block per pair of pair class c (0 when the
+// class needs no staging); implemented with the class dispatch
+int schwarz_block(int pair_cls);
 
 void launch_jk(int bra_cls, int ket_cls, bool pure, const JKLaunch& a, hipStream_t s);
 void launch_schwarz(int pair_cls, bool pure, const SchwarzLaunch& a, hipStream_t s);

@@ -137,6 +137,27 @@ class RysERI:
         return {k: getattr(s, k) for k, _ in _lib.QfeStats._fields_}
 
 
+class ERIMultiplexer:
+    """Routes J and K to one engine, or J and K to two engines (J with omega 0),
+    and gradients to a third when given; restates ERIMultiplexer
+    (eri_multiplexer.cpp:34-71): a single engine is called once for both."""
+
+    def __init__(self, j_engine, k_engine=None, grad_engine=None):
+        self.j_impl = j_engine
+        self.k_impl = k_engine if k_engine is not None else j_engine
+        self.grad_impl = grad_engine if grad_engine is not None else j_engine
+
+    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0):
+        if self.j_impl is self.k_impl:
+            return self.j_impl.build_JK(P, alpha, beta, omega)
+        J, _ = self.j_impl.build_JK(P, alpha, beta, 0.0)
+        _, K = self.k_impl.build_JK(P, alpha, beta, omega)
+        return J, K
+
+    def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0):
+        return self.grad_impl.get_gradients(P, alpha, beta, omega)
+
+
 def compiled_max_l() -> int:
     """Highest shell angular momentum the ERI kernels of this build handle."""
     L = _lib.lib()

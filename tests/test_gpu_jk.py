@@ -66,9 +66,10 @@ def test_mixed_precision_jk_close_to_oracle():
 
 
 @pytest.mark.parametrize("mix", [False, True])
-def test_segment_and_per_task_digestion_agree(monkeypatch, mix):
-    """The LDS segment digestion (default) and the per-task atomic digestion
-    (QFE_SEGMENT_DIGEST=0) give the same J and K, and both match the oracle."""
+def test_segment_digestion_variants_agree(monkeypatch, mix):
+    """The segment kernel with exchange and density rows in LDS (default) and
+    with exchange rows only (QFE_SEGMENT_DIGEST=0, density read from L2) give
+    the same J and K, and both match the oracle."""
     from oracle.oracle import OracleERI
     from qdk_chemistry_amd.basis import load_basis
     from qdk_chemistry_amd.eri import ERIConfig, create
@@ -80,7 +81,7 @@ def test_segment_and_per_task_digestion_agree(monkeypatch, mix):
     J1, K1 = create(b, mol, cfg).build_JK(P)
     monkeypatch.setenv("QFE_SEGMENT_DIGEST", "0")
     J0, K0 = create(b, mol, cfg).build_JK(P)
-    print("segment vs per-task: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
+    print("K+P rows vs K rows: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
           flush=True)
     assert np.max(np.abs(J1 - J0)) < 1e-11
     assert np.max(np.abs(K1 - K0)) < 1e-11

@@ -144,3 +144,25 @@ def test_c_abi_rejects_invalid_basis():
     assert b"positive" in L.qfe_last_error()
     rc = L.qfe_build_jk(None, _lib.ptr(np.zeros(1)), None, None, 1.0, 0.0, 0.0, 1)
     assert rc == _lib.QFE_ERR_RUNTIME
+
+
+def test_eri_multiplexer_routes_like_the_reference():
+    """One engine: called once for J and K; two engines: J from the first with
+    omega 0, K from the second (eri_multiplexer.cpp:60-71)."""
+    from qdk_chemistry_amd.eri import ERIMultiplexer
+    calls = []
+
+    class Fake:
+        def __init__(self, tag):
+            self.tag = tag
+
+        def build_JK(self, P, alpha, beta, omega):
+            calls.append((self.tag, omega))
+            return np.full_like(P, 1.0 if self.tag == "j" else 2.0), np.full_like(P, 3.0 if self.tag == "j" else 4.0)
+
+    P = np.zeros((2, 2))
+    J, K = ERIMultiplexer(Fake("j")).build_JK(P, 1.0, 0.0, 0.3)
+    assert calls == [("j", 0.3)] and J[0, 0] == 1.0 and K[0, 0] == 3.0
+    calls.clear()
+    J, K = ERIMultiplexer(Fake("j"), Fake("k")).build_JK(P, 1.0, 0.0, 0.3)
+    assert calls == [("j", 0.0), ("k", 0.3)] and J[0, 0] == 1.0 and K[0, 0] == 4.0
