@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 
 #include "eri_core.hpp"
@@ -102,8 +103,8 @@ struct SegRows {
   static constexpr int NR = NMA + NMB;  // LDS rows: a functions, then b functions
 };
 
-template <class C, int Q0, int Q1, typename R, bool PROW, int K0, int K1>
-__global__ __launch_bounds__(256) void k_eri_jk_seg(const int2* __restrict__ segs,
+template <class C, int Q0, int Q1, typename R, bool PROW, int K0, int K1, int BLK>
+__global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ segs,
                                                     const long long* __restrict__ offs,
                                                     const int2* __restrict__ tasks,
                                                     const PairHdr* __restrict__ pairs,
@@ -122,6 +123,8 @@ __global__ __launch_bounds__(256) void k_eri_jk_seg(const int2* __restrict__ seg
   const int i0 = bra.bfa, j0 = bra.bfb;
   double* Ks = lds;
   const double* Pa = PROW ? lds + (size_t)S::NR * nbf : nullptr;
+  __shared__ double sjab[NQ];  // J_ab of the block's bra functions
+  if (threadIdx.x < NQ) sjab[threadIdx.x] = 0.0;
   for (int r = 0; r < S::NR; ++r) {
     const size_t ao = r < S::NMA ? (size_t)(i0 + S::MA0 + r) : (size_t)(j0 + S::MB_LO + (r - S::NMA));
     for (int c = threadIdx.x; c < nbf; c += blockDim.x) {
@@ -204,15 +207,20 @@ __global__ __launch_bounds__(256) void k_eri_jk_seg(const int2* __restrict__ seg
       });
     }
   }
-  // J_ab: every thread of the block holds partial sums of the same entries
+  // J_ab: every thread of the block holds partial sums of the same entries;
+  // waves reduce in registers, then through LDS to one global atomic each
   const int lane = (int)(threadIdx.x & 63);
   sfor<NQ>([&](auto qq) {
-    constexpr int qi = Q0 + qq;
     const double tot = wave_sum(jij[qq]);
-    if (lane == 0 && tot != 0.0) unsafeAtomicAdd(&J[(size_t)(i0 + qi / C::NSB) * n + j0 + qi % C::NSB], tot);
+    if (lane == 0 && tot != 0.0) atomicAdd(&sjab[qq], tot);
   });
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const int qi = Q0 + (int)threadIdx.x;
+    const double v = sjab[threadIdx.x];
+    if (v != 0.0) unsafeAtomicAdd(&J[(size_t)(i0 + qi / C::NSB) * n + j0 + qi % C::NSB], v);
+  }
   if (kscale != 0.0) {
-    __syncthreads();
     const double kq = 0.25 * kscale;
     for (int r = 0; r < S::NR; ++r) {
       const size_t ao = r < S::NMA ? (size_t)(i0 + S::MA0 + r) : (size_t)(j0 + S::MB_LO + (r - S::NMA));
@@ -295,43 +303,65 @@ struct ClassOps {
   using C = Cls<LA, LB, LC, LD, PURE>;
   using K = JKChunks<C>;
   static constexpr int NCH = K::NCH;
+  // threads per segment block: the LDS rows of one bra pair are shared by
+  // all of them, so light classes use large blocks to keep enough waves per
+  // CU resident (the register budget per thread shrinks with the block)
+  static constexpr int SEG_BLK =
+      K::F ? 256 : ((C::L <= 2 && C::NBRA <= 5) ? 1024 : (C::L <= 3 ? 512 : 256));
   // doubles of the Schwarz staging block per pair (classes with f shells)
   static constexpr int SCHWARZ_BLOCK = K::F ? C::NBRA * C::NKETC : 0;
 
-  // LDS bytes of the segment kernel (K rows, and P rows with prow)
+  // dynamic LDS bytes of the segment kernel (K rows, and P rows with prow),
+  // plus its static J_ab slots when checked against the LDS limit
   template <int Q0, int Q1>
   static size_t seg_bytes(int nbf, bool prow) {
     return (size_t)SegRows<C, Q0, Q1>::NR * nbf * sizeof(double) * (prow ? 2 : 1);
   }
+  template <int Q0, int Q1>
+  static size_t seg_total(int nbf, bool prow) {
+    return seg_bytes<Q0, Q1>(nbf, prow) + (size_t)(Q1 - Q0) * sizeof(double);
+  }
 
   template <int Q0, int Q1, int K0, int K1, typename R, bool PROW>
   static bool launch_seg(const JKLaunch& a, hipStream_t s, size_t bytes) {
-    auto kern = k_eri_jk_seg<C, Q0, Q1, R, PROW, K0, K1>;
+    auto kern = k_eri_jk_seg<C, Q0, Q1, R, PROW, K0, K1, SEG_BLK>;
     if (bytes > 65536 &&
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
       (void)hipGetLastError();
       return false;
     }
     if constexpr (sizeof(R) == 4)
-      hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(256), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
+      hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
                          a.ppos32, a.pk32, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale);
     else
-      hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(256), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
+      hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
                          a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale);
     return true;
   }
 
   // segment kernel with K and P rows in LDS when they fit, else K rows only
   // (P from L2); a negative lds_max forces the K-only variant
+  // waves of this kernel one CU can hold when a block needs `bytes` of LDS
+  // (the CU's LDS equals the per-block maximum on gfx950; 32 wave slots)
+  static int waves_per_cu(size_t bytes, size_t lds) {
+    const size_t blocks_lds = bytes ? lds / bytes : 32;
+    const size_t blocks_slots = (size_t)32 / (SEG_BLK / 64);
+    return (int)std::min(blocks_lds, blocks_slots) * (SEG_BLK / 64);
+  }
+
   template <int Q0, int Q1, int K0, int K1, typename R, bool BOTH>
   static void seg(const JKLaunch& a, hipStream_t s) {
     static_assert(SegRows<C, Q0, Q1>::NR <= kSegRowsMax, "chunk needs more LDS rows than planned");
     if (a.nseg <= 0 || a.segs == nullptr) throw std::runtime_error("segment list missing");
     const size_t lim = (size_t)(a.lds_max < 0 ? -a.lds_max : a.lds_max);
     const size_t with_p = seg_bytes<Q0, Q1>(a.nbf, true), k_only = seg_bytes<Q0, Q1>(a.nbf, false);
-    if (a.lds_max > 0 && with_p <= lim && launch_seg<Q0, Q1, K0, K1, R, true>(a, s, with_p)) return;
+    const size_t tot_p = seg_total<Q0, Q1>(a.nbf, true), tot_k = seg_total<Q0, Q1>(a.nbf, false);
+    // density rows in LDS unless dropping them lets more waves reside
+    const bool prow = a.lds_max > 0 && tot_p <= lim &&
+                      (!BOTH || waves_per_cu(tot_p, lim) >= waves_per_cu(tot_k, lim));
+    if (prow && launch_seg<Q0, Q1, K0, K1, R, true>(a, s, with_p)) return;
     if constexpr (BOTH)
-      if (k_only <= lim && launch_seg<Q0, Q1, K0, K1, R, false>(a, s, k_only)) return;
+      if (tot_k <= lim && launch_seg<Q0, Q1, K0, K1, R, false>(a, s, k_only)) return;
     throw std::runtime_error("the exchange rows of one bra pair exceed the LDS of a block (nbf too large)");
   }
 
