@@ -37,6 +37,14 @@ thread_local std::string g_last_error;
 struct DeviceError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+// functions of a shell pair of pair class c = la (la + 1) / 2 + lb (pure)
+static int nfun_pair_class(int c) {
+  int la = 0;
+  while ((la + 1) * (la + 2) / 2 <= c) ++la;
+  const int lb = c - la * (la + 1) / 2;
+  return nsph(la, true) * nsph(lb, true);
+}
+
 struct Unsupported : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
@@ -196,6 +204,14 @@ struct Engine {
   std::vector<int> cp_seg_begin;     // per class pair: first segment (size ncp + 1)
   DevBuf<int2> segs_d;
   int lds_max = 65536;               // dynamic LDS bytes a block may request
+  // J_cd by ket pair: per-task shares, sort keys/values (two halves each),
+  // run bounds per pair, radix-sort workspace
+  bool jcd_sort = true;
+  std::vector<int> cls_pair_begin;  // first pair of each pair class (pairs are class-sorted)
+  DevBuf<double> jbuf_d;
+  DevBuf<unsigned> jkeys_d, jvals_d;
+  DevBuf<int> jruns_d;
+  DevBuf<unsigned char> jsort_tmp_d;
   std::vector<int> cp_bra, cp_ket;   // class pair -> (bra class, ket class)
   long long prim_pairs = 0;
 
@@ -474,6 +490,12 @@ struct Engine {
     cp_item_begin[ncp] = (int)items_h.size();
     cp_seg_begin[ncp] = (int)segs_h.size();
 
+    {
+      const int ncls_all = (QFE_MAX_L + 1) * (QFE_MAX_L + 2) / 2;
+      cls_pair_begin.assign(ncls_all + 1, npair);
+      for (int i = npair - 1; i >= 0; --i) cls_pair_begin[pairs_h[i].cls] = i;
+      for (int c = ncls_all - 1; c >= 0; --c) cls_pair_begin[c] = std::min(cls_pair_begin[c], cls_pair_begin[c + 1]);
+    }
     pairs_d.upload(pairs_h, stream);
     std::vector<int2> psh(npair);
     for (int i = 0; i < npair; ++i) psh[i] = make_int2(pairs_h[i].sa, pairs_h[i].sb);
@@ -548,6 +570,9 @@ struct Engine {
     // rows are read from L2), the variant used when both do not fit
     if (const char* e = std::getenv("QFE_SEGMENT_DIGEST"))
       if (e[0] == '0') lds_max = -lds_max;
+    // QFE_JCD_SORT=0 adds every task's J_cd share to J with global atomics
+    if (const char* e = std::getenv("QFE_JCD_SORT"))
+      if (e[0] == '0') jcd_sort = false;
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     setup_rys();
@@ -676,6 +701,28 @@ struct Engine {
       Kw = K_d.p;
       HIP_CHECK(hipMemsetAsync(Kw, 0, nn * sizeof(double), s));
     }
+    // J_cd shares per task, summed per ket pair after the class kernels, when
+    // the task list fits the 32-bit sort keys and offsets
+    const long long ntask_all = bval[2 * (ncp + 1) - 1];
+    long long jsize = 0;
+    for (int prec = 0; prec < 2; ++prec) {
+      const long long* b = bval.data() + prec * (ncp + 1);
+      for (int cp = 0; cp < ncp; ++cp) jsize += (b[cp + 1] - b[cp]) * (long long)nfun_pair_class(cp_ket[cp]);
+    }
+    const int npair = (int)pairs_h.size();
+    const bool jsort = jcd_sort && ntask_all > 0 && ntask_all < (1LL << 31) - 1 && jsize < (1LL << 32);
+    int jbits = 1;
+    while ((1 << jbits) < npair) ++jbits;
+    size_t jtmp = 0;
+    if (jsort) {
+      jbuf_d.alloc((size_t)jsize);
+      jkeys_d.alloc(2 * (size_t)ntask_all);
+      jvals_d.alloc(2 * (size_t)ntask_all);
+      jruns_d.alloc(2 * (size_t)npair);
+      jtmp = jcd_sort_temp_bytes(ntask_all, jbits);
+      jsort_tmp_d.alloc(jtmp);
+    }
+    long long jbase = 0;
     long long nq = 0, nq32 = 0;
     for (int prec = 0; prec < 2; ++prec) {
       const long long* b = bval.data() + prec * (ncp + 1);
@@ -686,13 +733,21 @@ struct Engine {
         JKLaunch a{tasks_d.p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jw, Kw, nbf, kscale,
                    prec == 1, ppos32_d.p, pk32_d.p, &ra32,
                    segs_d.p + cp_seg_begin[cp], cp_seg_begin[cp + 1] - cp_seg_begin[cp],
-                   offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_d.p, lds_max};
+                   offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_d.p, lds_max,
+                   jsort ? jbuf_d.p : nullptr, b[cp], jbase, jkeys_d.p, jvals_d.p};
+        jbase += n * (long long)nfun_pair_class(cp_ket[cp]);
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
         if (e) HIP_CHECK(hipEventRecord(e[0], s));
         launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
         HIP_CHECK(hipGetLastError());
         if (e) HIP_CHECK(hipEventRecord(e[1], s));
       }
+    }
+    if (jsort) {
+      launch_jcd_reduce(jsort_tmp_d.p, jtmp, jkeys_d.p, jkeys_d.p + ntask_all, jvals_d.p, jvals_d.p + ntask_all,
+                        ntask_all, jbits, pairs_d.p, npair, cls_pair_begin.data(), (int)cls_pair_begin.size() - 1,
+                        jruns_d.p, jruns_d.p + npair, jbuf_d.p, Jw, nbf, s);
+      HIP_CHECK(hipGetLastError());
     }
     HIP_CHECK(hipEventRecord(ev[2], s));
     launch_symmetrize(dJ, dK, nbf, s);

@@ -111,7 +111,10 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
                                                     const typename Vec4<R>::type* __restrict__ ppos,
                                                     const R* __restrict__ pk, RysArgsT<R> ra,
                                                     const double* __restrict__ P, double* __restrict__ J,
-                                                    double* __restrict__ K, int nbf, double kscale) {
+                                                    double* __restrict__ K, int nbf, double kscale,
+                                                    double* __restrict__ jbuf, long long jrange0, long long jbase,
+                                                    unsigned* __restrict__ jkeys, unsigned* __restrict__ jvals,
+                                                    int jfirst) {
   using S = SegRows<C, Q0, Q1>;
   extern __shared__ double lds[];
   const int2 sg = segs[blockIdx.x];
@@ -191,9 +194,21 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
         });
       });
     });
-    sfor<C::NSC>([&](auto c) {
-      sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&J[(size_t)(k0 + c) * n + l0 + d], jkl[c * C::NSD + d]); });
-    });
+    if (jbuf) {
+      // J_cd share of this task, summed per ket pair after the class kernels
+      double* jt = jbuf + jbase + (size_t)(t - jrange0) * C::NKETS;
+      if (jfirst) {
+        sfor<C::NKETS>([&](auto f) { jt[f] = jkl[f]; });
+        jkeys[t] = (unsigned)tk.y;
+        jvals[t] = (unsigned)(jbase + (t - jrange0) * C::NKETS);
+      } else {
+        sfor<C::NKETS>([&](auto f) { jt[f] += jkl[f]; });
+      }
+    } else {
+      sfor<C::NSC>([&](auto c) {
+        sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&J[(size_t)(k0 + c) * n + l0 + d], jkl[c * C::NSD + d]); });
+      });
+    }
     if (kscale != 0.0) {
       sfor<S::NMA>([&](auto a) {
         double* row = Ks + (size_t)a * nbf;
@@ -330,12 +345,15 @@ struct ClassOps {
       (void)hipGetLastError();
       return false;
     }
+    const int jfirst = (Q0 == 0 && K0 == 0) ? 1 : 0;  // chunk 0 of the class runs first
     if constexpr (sizeof(R) == 4)
       hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
-                         a.ppos32, a.pk32, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale);
+                         a.ppos32, a.pk32, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase,
+                         a.jkeys, a.jvals, jfirst);
     else
       hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
-                         a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale);
+                         a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase, a.jkeys,
+                         a.jvals, jfirst);
     return true;
   }
 

@@ -242,6 +242,107 @@ void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_symmetrize, grid, block, 0, s, J, K, n);
 }
 
+// ------------------------------------------------------------ J_cd by ket pair
+__global__ void k_run_bounds(const unsigned* __restrict__ keys, long long n, int* __restrict__ run_start,
+                             int* __restrict__ run_end) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned k = keys[i];
+  if (i == 0 || keys[i - 1] != k) run_start[k] = (int)i;
+  if (i == n - 1 || keys[i + 1] != k) run_end[k] = (int)(i + 1);
+}
+
+__device__ __forceinline__ double wave_sum64(double v) {
+  v += __shfl_xor(v, 32);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 1);
+  return v;
+}
+
+// one wave per ket pair of one pair class (NK functions per pair): each lane
+// adds whole shares of the run (NK contiguous doubles), then the wave sums
+// them and adds one atomic per J element
+template <int NK>
+__global__ __launch_bounds__(256) void k_jcd_sum(const PairHdr* __restrict__ pairs, int q0, int q1, int nsb,
+                                                 const int* __restrict__ run_start, const int* __restrict__ run_end,
+                                                 const unsigned* __restrict__ vals, const double* __restrict__ jbuf,
+                                                 double* __restrict__ J, int nbf) {
+  const int q = q0 + (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  const int lane = (int)(threadIdx.x & 63);
+  if (q >= q1) return;  // whole waves
+  const int i0 = run_start[q], i1 = run_end[q];
+  if (i0 >= i1) return;
+  double acc[NK];
+#pragma unroll
+  for (int f = 0; f < NK; ++f) acc[f] = 0.0;
+  for (int i = i0 + lane; i < i1; i += 64) {
+    const double* sh = jbuf + vals[i];
+#pragma unroll
+    for (int f = 0; f < NK; ++f) acc[f] += sh[f];
+  }
+  const PairHdr h = pairs[q];
+#pragma unroll
+  for (int f = 0; f < NK; ++f) {
+    const double v = wave_sum64(acc[f]);
+    if (lane == 0 && v != 0.0) unsafeAtomicAdd(&J[(size_t)(h.bfa + f / nsb) * nbf + h.bfb + f % nsb], v);
+  }
+}
+
+template <int NK>
+static void jcd_sum_class(const PairHdr* pairs, int q0, int q1, int nsb, const int* run_start, const int* run_end,
+                          const unsigned* vals, const double* jbuf, double* J, int nbf, hipStream_t s) {
+  const long long threads = (long long)(q1 - q0) * 64;
+  hipLaunchKernelGGL((k_jcd_sum<NK>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, pairs, q0, q1, nsb,
+                     run_start, run_end, vals, jbuf, J, nbf);
+}
+
+size_t jcd_sort_temp_bytes(long long n, int bits) {
+  size_t bytes = 0;
+  hipcub::DoubleBuffer<unsigned> k(nullptr, nullptr), v(nullptr, nullptr);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)n, 0, bits);
+  return bytes;
+}
+
+void launch_jcd_reduce(void* temp, size_t temp_bytes, unsigned* keys0, unsigned* keys1, unsigned* vals0,
+                       unsigned* vals1, long long n, int bits, const PairHdr* pairs, int npair,
+                       const int* cls_begin, int ncls, int* run_start, int* run_end, const double* jbuf, double* J,
+                       int nbf, hipStream_t s) {
+  if (n <= 0) return;
+  hipcub::DoubleBuffer<unsigned> k(keys0, keys1), v(vals0, vals1);
+  (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k, v, (int)n, 0, bits, s);
+  (void)hipMemsetAsync(run_start, 0, sizeof(int) * npair, s);
+  (void)hipMemsetAsync(run_end, 0, sizeof(int) * npair, s);
+  hipLaunchKernelGGL(k_run_bounds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, k.Current(), n, run_start,
+                     run_end);
+  // pairs are sorted by class: one launch per pair class, templated on its functions
+  for (int c = 0; c < ncls; ++c) {
+    const int q0 = cls_begin[c], q1 = cls_begin[c + 1];
+    if (q0 >= q1) continue;
+    int la = 0;
+    while ((la + 1) * (la + 2) / 2 <= c) ++la;
+    const int lb = c - la * (la + 1) / 2;
+    const int nsa = la >= 2 ? 2 * la + 1 : (la + 1) * (la + 2) / 2;
+    const int nsb = lb >= 2 ? 2 * lb + 1 : (lb + 1) * (lb + 2) / 2;
+    const unsigned* vs = v.Current();
+    switch (nsa * nsb) {
+      case 1: jcd_sum_class<1>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 3: jcd_sum_class<3>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 5: jcd_sum_class<5>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 7: jcd_sum_class<7>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 9: jcd_sum_class<9>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 15: jcd_sum_class<15>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 21: jcd_sum_class<21>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 25: jcd_sum_class<25>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 35: jcd_sum_class<35>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      case 49: jcd_sum_class<49>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
+      default: break;  // no other pure pair sizes up to f
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_copy_words(const unsigned long long* __restrict__ src,
                                                     unsigned long long* __restrict__ dst, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;

@@ -39,6 +39,14 @@ struct JKLaunch {
   const long long* offs;
   const int2* tasks_all;
   int lds_max;        // dynamic LDS bytes a block may request (negative: exchange rows only)
+  // J_cd by ket pair (jbuf null: one global atomic per task function pair):
+  // task t of this launch writes its share to jbuf[jbase + (t - jrange0) * NKETS
+  // + f]; the first chunk of the class stores it with jkeys[t] = ket pair and
+  // jvals[t] = the offset, later chunks add to it
+  double* jbuf;
+  long long jrange0, jbase;
+  unsigned* jkeys;
+  unsigned* jvals;
 };
 
 struct GradLaunch {

@@ -30,6 +30,21 @@ void launch_class_stats(const int* counts, const int2* items, long long nitems, 
                         hipStream_t s);
 void launch_symmetrize(double* J, double* K, int n, hipStream_t s);
 
+// J_cd by ket pair: the class kernels write each task's J_cd share to jbuf
+// (NKETS doubles at a per-task offset) with key = ket pair and value = that
+// offset; the keys are sorted and every ket pair's run is summed into J with
+// one atomic per J element.  jcd_sort_temp_bytes sizes the radix-sort
+// workspace for n keys of `bits` bits; launch_jcd_reduce sorts (double
+// buffers keys0/keys1, vals0/vals1), finds each pair's run (run_start,
+// run_end: npair ints, cleared here) and adds the sums to J.
+size_t jcd_sort_temp_bytes(long long n, int bits);
+// cls_begin (host, ncls + 1 entries): first pair of each pair class (pairs
+// are sorted by class)
+void launch_jcd_reduce(void* temp, size_t temp_bytes, unsigned* keys0, unsigned* keys1, unsigned* vals0,
+                       unsigned* vals1, long long n, int bits, const PairHdr* pairs, int npair,
+                       const int* cls_begin, int ncls, int* run_start, int* run_end, const double* jbuf, double* J,
+                       int nbf, hipStream_t s);
+
 // plain device-to-device copy of n 8-byte words (also the reference kernel for
 // HBM traffic counters: it moves exactly 8n bytes each way)
 void launch_copy_words(const unsigned long long* src, unsigned long long* dst, size_t n, hipStream_t s);

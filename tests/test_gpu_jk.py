@@ -89,3 +89,45 @@ def test_segment_digestion_variants_agree(monkeypatch, mix):
         Jr, Kr = OracleERI(b, mol).build_JK(P)
         assert np.max(np.abs(J1 - Jr)) < 1e-10
         assert np.max(np.abs(K1 - Kr)) < 1e-10
+
+
+@pytest.mark.parametrize("mix", [False, True])
+def test_jcd_sorted_and_atomic_paths_agree(monkeypatch, mix):
+    """J_cd summed per ket pair after a sort (default) and added per task with
+    global atomics (QFE_JCD_SORT=0) give the same J; both match the oracle."""
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 13)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = random_density(b.nbf, 8, 0.3)
+    cfg = ERIConfig(is_mix_precision=mix)
+    J1, K1 = create(b, mol, cfg).build_JK(P)
+    monkeypatch.setenv("QFE_JCD_SORT", "0")
+    J0, K0 = create(b, mol, cfg).build_JK(P)
+    print("sorted vs atomic J_cd: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
+          flush=True)
+    assert np.max(np.abs(J1 - J0)) < 1e-11
+    assert np.max(np.abs(K1 - K0)) < 1e-11
+    if not mix:
+        Jr, Kr = OracleERI(b, mol).build_JK(P)
+        assert np.max(np.abs(J1 - Jr)) < 1e-10
+        assert np.max(np.abs(K1 - Kr)) < 1e-10
+
+
+def test_jk_repeatable():
+    """Two builds of the same density agree to rounding (atomic additions
+    make the order of the last bits vary)."""
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 13)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = random_density(b.nbf, 9, 0.3)
+    eng = create(b, mol)
+    J1, K1 = eng.build_JK(P)
+    J2, K2 = eng.build_JK(P)
+    print("repeat: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J2)), np.max(np.abs(K1 - K2))), flush=True)
+    assert np.max(np.abs(J1 - J2)) < 1e-13
+    assert np.max(np.abs(K1 - K2)) < 1e-13
