@@ -207,6 +207,7 @@ struct Engine {
   // J_cd by ket pair: per-task shares, sort keys/values (two halves each),
   // run bounds per pair, radix-sort workspace
   bool jcd_sort = true;
+  int seg_min_waves = 8;
   std::vector<int> cls_pair_begin;  // first pair of each pair class (pairs are class-sorted)
   DevBuf<double> jbuf_d;
   DevBuf<unsigned> jkeys_d, jvals_d;
@@ -570,6 +571,8 @@ struct Engine {
     // rows are read from L2), the variant used when both do not fit
     if (const char* e = std::getenv("QFE_SEGMENT_DIGEST"))
       if (e[0] == '0') lds_max = -lds_max;
+    // QFE_SEG_WAVES: resident waves per CU below which the density rows leave LDS
+    if (const char* e = std::getenv("QFE_SEG_WAVES")) seg_min_waves = std::max(1, std::atoi(e));
     // QFE_JCD_SORT=0 adds every task's J_cd share to J with global atomics
     if (const char* e = std::getenv("QFE_JCD_SORT"))
       if (e[0] == '0') jcd_sort = false;
@@ -734,7 +737,7 @@ struct Engine {
                    prec == 1, ppos32_d.p, pk32_d.p, &ra32,
                    segs_d.p + cp_seg_begin[cp], cp_seg_begin[cp + 1] - cp_seg_begin[cp],
                    offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_d.p, lds_max,
-                   jsort ? jbuf_d.p : nullptr, b[cp], jbase, jkeys_d.p, jvals_d.p};
+                   jsort ? jbuf_d.p : nullptr, b[cp], jbase, jkeys_d.p, jvals_d.p, seg_min_waves};
         jbase += n * (long long)nfun_pair_class(cp_ket[cp]);
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
         if (e) HIP_CHECK(hipEventRecord(e[0], s));

@@ -52,7 +52,7 @@ constexpr int seg_bch(int hi) {
 // shell hold fewer and are also chunked over the ket cartesians, so that no
 // kernel instance spills registers.
 constexpr int kAccMax = 48;
-constexpr int kAccMaxF = 20;
+constexpr int kAccMaxF = 16;
 template <class C>
 struct JKChunks {
   static constexpr bool F = C::LA > 2 || C::LB > 2 || C::LC > 2 || C::LD > 2;
@@ -114,11 +114,15 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
                                                     double* __restrict__ K, int nbf, double kscale,
                                                     double* __restrict__ jbuf, long long jrange0, long long jbase,
                                                     unsigned* __restrict__ jkeys, unsigned* __restrict__ jvals,
-                                                    int jfirst) {
+                                                    int jfirst, int nsplit) {
   using S = SegRows<C, Q0, Q1>;
   extern __shared__ double lds[];
-  const int2 sg = segs[blockIdx.x];
-  const long long t0 = offs[sg.x], t1 = offs[sg.y];
+  // block = (segment, part): a long segment may be split over nsplit blocks,
+  // each with its own LDS rows, when there are too few segments for the GPU
+  const int2 sg = segs[blockIdx.x / nsplit];
+  const int part = (int)(blockIdx.x % nsplit);
+  const long long s0 = offs[sg.x], s1 = offs[sg.y];
+  const long long t0 = s0 + (s1 - s0) * part / nsplit, t1 = s0 + (s1 - s0) * (part + 1) / nsplit;
   if (t0 >= t1) return;  // the same for every thread of the block
   const PairHdr bra = pairs[tasks[t0].x];
   constexpr int NQ = Q1 - Q0;
@@ -346,14 +350,20 @@ struct ClassOps {
       return false;
     }
     const int jfirst = (Q0 == 0 && K0 == 0) ? 1 : 0;  // chunk 0 of the class runs first
+    // split segments when there are fewer than ~2048 blocks and the average
+    // segment gives every thread several tasks
+    const long long per = a.ntask / std::max(a.nseg, 1);
+    const long long want = (2048 + a.nseg - 1) / std::max(a.nseg, 1);
+    const int nsplit = (int)std::max(1LL, std::min({want, per / (2 * SEG_BLK), 16LL}));
+    const dim3 grid((unsigned)((long long)a.nseg * nsplit));
     if constexpr (sizeof(R) == 4)
-      hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
-                         a.ppos32, a.pk32, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase,
-                         a.jkeys, a.jvals, jfirst);
+      hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos32, a.pk32,
+                         *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase, a.jkeys, a.jvals,
+                         jfirst, nsplit);
     else
-      hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs,
-                         a.ppos, a.pk, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase, a.jkeys,
-                         a.jvals, jfirst);
+      hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos, a.pk,
+                         *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase, a.jkeys, a.jvals, jfirst,
+                         nsplit);
     return true;
   }
 
@@ -374,9 +384,10 @@ struct ClassOps {
     const size_t lim = (size_t)(a.lds_max < 0 ? -a.lds_max : a.lds_max);
     const size_t with_p = seg_bytes<Q0, Q1>(a.nbf, true), k_only = seg_bytes<Q0, Q1>(a.nbf, false);
     const size_t tot_p = seg_total<Q0, Q1>(a.nbf, true), tot_k = seg_total<Q0, Q1>(a.nbf, false);
-    // density rows in LDS unless dropping them lets more waves reside
-    const bool prow = a.lds_max > 0 && tot_p <= lim &&
-                      (!BOTH || waves_per_cu(tot_p, lim) >= waves_per_cu(tot_k, lim));
+    // density rows in LDS unless that leaves fewer than seg_min_waves waves
+    // per CU and dropping them lets more reside
+    const int wp = waves_per_cu(tot_p, lim), wk = waves_per_cu(tot_k, lim);
+    const bool prow = a.lds_max > 0 && tot_p <= lim && (!BOTH || wp >= wk || wp >= a.seg_min_waves);
     if (prow && launch_seg<Q0, Q1, K0, K1, R, true>(a, s, with_p)) return;
     if constexpr (BOTH)
       if (tot_k <= lim && launch_seg<Q0, Q1, K0, K1, R, false>(a, s, k_only)) return;

@@ -47,6 +47,9 @@ struct JKLaunch {
   long long jrange0, jbase;
   unsigned* jkeys;
   unsigned* jvals;
+  // keep the density rows in LDS when that still leaves at least this many
+  // resident waves per CU (else the K-row-only variant, if it fits more)
+  int seg_min_waves;
 };
 
 struct GradLaunch {
