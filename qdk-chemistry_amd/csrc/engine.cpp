@@ -18,6 +18,7 @@
 #include <cstring>
 #include <dlfcn.h>
 #include <fstream>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -204,20 +205,32 @@ struct Engine {
   std::vector<int> cp_seg_begin;     // per class pair: first segment (size ncp + 1)
   DevBuf<int2> segs_d;
   int lds_max = 65536;               // dynamic LDS bytes a block may request
-  // J_cd by ket pair: per-task shares, sort keys/values (two halves each),
-  // run bounds per pair, radix-sort workspace
-  bool jcd_sort = true;
+  // J_cd by ket pair: share slots per item (counts nslot, bases jslot, both
+  // nitems + 1 long), the slots themselves, per pair (ket bucket relative to
+  // the first bucket of its class, position in its bucket), functions per
+  // ket pair of each class pair, and the reduction tiles grouped by ket pair
+  // size: tiles [begin, begin + count) of group g have nk, nsb = jt_groups[g]
+  bool jcd_slots = true;
   int seg_min_waves = 8;
   std::vector<int> cls_pair_begin;  // first pair of each pair class (pairs are class-sorted)
+  DevBuf<int> nslot_d;
+  DevBuf<long long> jslot_d;
+  DevBuf<long long> sums_d;  // 64-bit sums of the task counts and of the slot counts
   DevBuf<double> jbuf_d;
-  DevBuf<unsigned> jkeys_d, jvals_d;
-  DevBuf<int> jruns_d;
-  DevBuf<unsigned char> jsort_tmp_d;
+  DevBuf<int2> qpos_d;
+  DevBuf<int> cp_nk_d;
+  DevBuf<int4> jtiles_d;
+  struct JTileGroup {
+    int nk, nsb, begin, count;
+  };
+  std::vector<JTileGroup> jt_groups;
+  long long jslot_total = 0;
   std::vector<int> cp_bra, cp_ket;   // class pair -> (bra class, ket class)
   long long prim_pairs = 0;
 
   DevBuf<PairHdr> pairs_d;
   DevBuf<int2> psh_d;  // (sa, sb) per sorted pair, for the screening scan
+  DevBuf<float> Pcd_d;  // per pair: density norm of its shell block, per build
   DevBuf<double4> ppos_d;
   DevBuf<double> pk_d;
   DevBuf<double> pa_d;      // exponent of each primitive pair's first shell (gradients)
@@ -501,6 +514,7 @@ struct Engine {
     std::vector<int2> psh(npair);
     for (int i = 0; i < npair; ++i) psh[i] = make_int2(pairs_h[i].sa, pairs_h[i].sb);
     psh_d.upload(psh, stream);
+    Pcd_d.alloc(npair);
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
     pa_d.upload(pa2, stream);
@@ -528,6 +542,12 @@ struct Engine {
     offsets_d.alloc(2 * nitems + 1);
     scan_tmp_bytes = scan_temp_bytes(2 * nitems + 1);
     scan_tmp_d.alloc(scan_tmp_bytes);
+    nslot_d.alloc(nitems + 1);
+    HIP_CHECK(hipMemsetAsync(nslot_d.p, 0, sizeof(int) * (nitems + 1), stream));
+    jslot_d.alloc(nitems + 1);
+    sums_d.alloc(2);
+    HIP_CHECK(hipMemsetAsync(sums_d.p, 0, sizeof(long long) * 2, stream));
+    build_jcd_tiles();
     // class-pair boundaries in the scanned counts: FP64 part, then FP32 part
     std::vector<long long> bidx(2 * (ncp + 1));
     for (int cp = 0; cp <= ncp; ++cp) {
@@ -540,6 +560,77 @@ struct Engine {
     if (verbose)
       fprintf(stderr, "[qfe] nsh=%d nbf=%d pairs=%d prim_pairs=%lld buckets=%d items=%lld\n", nsh, nbf,
               npair, prim_pairs, nb, nitems);
+  }
+
+  // Share-slot geometry of the J_cd reduction.  Within class pair cp, the
+  // segment of bra pair p (class cb) is cp_seg_begin[cp] + p - first pair of
+  // cb, and its items are the ket buckets of class ck from the first one,
+  // up to p's own bucket when cb == ck: every pair of one bra bucket has the
+  // same number of items, so the items (p, kb) of a bra bucket are evenly
+  // strided.  Tiles cover (bra bucket, ket bucket) in blocks of kJcdTileBra
+  // bra pairs x kJcdTileKet ket positions.
+  void build_jcd_tiles() {
+    const int npair = (int)pairs_h.size();
+    const int nb = (int)buckets_h.size();
+    const int ncls_all = (int)cls_pair_begin.size() - 1;
+    std::vector<int> cls_bucket0(ncls_all + 1, nb);
+    for (int b = nb - 1; b >= 0; --b) cls_bucket0[bucket_cls_h[b]] = b;
+    for (int c = ncls_all - 1; c >= 0; --c) cls_bucket0[c] = std::min(cls_bucket0[c], cls_bucket0[c + 1]);
+    std::vector<int2> qpos(npair);
+    for (int q = 0; q < npair; ++q) {
+      const int kb = pair_bucket_h[q];
+      qpos[q] = make_int2(kb - cls_bucket0[bucket_cls_h[kb]], q - buckets_h[kb].x);
+    }
+    qpos_d.upload(qpos, stream);
+    const int ncp = (int)cp_bra.size();
+    std::vector<int> cp_nk(ncp);
+    for (int cp = 0; cp < ncp; ++cp) cp_nk[cp] = nfun_pair_class(cp_ket[cp]);
+    cp_nk_d.upload(cp_nk, stream);
+    std::map<int, std::vector<int4>> by_nk;
+    std::map<int, int> nsb_of;
+    for (int cp = 0; cp < ncp; ++cp) {
+      const int cb = cp_bra[cp], ck = cp_ket[cp];
+      const int kbf = cls_bucket0[ck], kbl = cls_bucket0[ck + 1];  // ket buckets [kbf, kbl)
+      const int bbf = cls_bucket0[cb], bbl = cls_bucket0[cb + 1];
+      if (kbf >= kbl || bbf >= bbl) continue;
+      int la = 0;
+      while ((la + 1) * (la + 2) / 2 <= ck) ++la;
+      const int lb = ck - la * (la + 1) / 2;
+      const int nk = cp_nk[cp];
+      nsb_of[nk] = nsph(lb, true);
+      std::vector<int4>& out = by_nk[nk];
+      for (int kb = kbf; kb < kbl; ++kb) {
+        const int2 kbk = buckets_h[kb];
+        const int ksize = kbk.y - kbk.x;
+        for (int bb = std::max(bbf, kb); bb < bbl; ++bb) {
+          const int2 bbk = buckets_h[bb];
+          const int stride = (cb == ck) ? bb - kbf + 1 : kbl - kbf;
+          const int s0 = cp_seg_begin[cp] + (bbk.x - cls_pair_begin[cb]);
+          const int item0 = segs_h[s0].x + (kb - kbf);
+          if (stride >= (1 << 15))
+            throw std::runtime_error("too many pair buckets in one class for the J_cd tiles");
+          // the strided layout must name the items built above
+          for (int p : {bbk.x, bbk.y - 1}) {
+            const int2 it = items_h[item0 + (p - bbk.x) * stride];
+            if (it.x != p || it.y != kb) throw std::runtime_error("J_cd tile layout does not match the work items");
+          }
+          for (int p0 = bbk.x; p0 < bbk.y; p0 += kJcdTileBra) {
+            const int np = std::min(kJcdTileBra, bbk.y - p0);
+            // same bucket: bra pair p holds positions j <= p - first pair only
+            const int jmax = (bb == kb) ? std::min(ksize, p0 + np - kbk.x) : ksize;
+            for (int j0 = 0; j0 < jmax; j0 += kJcdTileKet)
+              out.push_back(make_int4(item0 + (p0 - bbk.x) * stride, (stride << 16) | np, kbk.x, j0));
+          }
+        }
+      }
+    }
+    std::vector<int4> all;
+    jt_groups.clear();
+    for (auto& kv : by_nk) {
+      jt_groups.push_back({kv.first, nsb_of[kv.first], (int)all.size(), (int)kv.second.size()});
+      all.insert(all.end(), kv.second.begin(), kv.second.end());
+    }
+    jtiles_d.upload(all, stream);
   }
 
   void create(const qfe_basis* b, const qfe_config* cfg) {
@@ -573,9 +664,9 @@ struct Engine {
       if (e[0] == '0') lds_max = -lds_max;
     // QFE_SEG_WAVES: resident waves per CU below which the density rows leave LDS
     if (const char* e = std::getenv("QFE_SEG_WAVES")) seg_min_waves = std::max(1, std::atoi(e));
-    // QFE_JCD_SORT=0 adds every task's J_cd share to J with global atomics
-    if (const char* e = std::getenv("QFE_JCD_SORT"))
-      if (e[0] == '0') jcd_sort = false;
+    // QFE_JCD_SLOTS=0 adds every task's J_cd share to J with global atomics
+    if (const char* e = std::getenv("QFE_JCD_SLOTS"))
+      if (e[0] == '0') jcd_slots = false;
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     setup_rys();
@@ -604,17 +695,30 @@ struct Engine {
 
   // Screened canonical task list for density dP (FP64 and FP32 parts);
   // returns the class-pair boundaries [2][ncp + 1] of the task array.
-  std::vector<long long> screen(const double* dP, hipStream_t s, bool allow_mix) {
+  std::vector<long long> screen(const double* dP, hipStream_t s, bool allow_mix, bool slots = false) {
     HIP_CHECK(hipMemsetAsync(bad_d.p, 0, sizeof(int), s));
     launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, Pnd_d.p, bad_d.p, pmax_d.p, s);
     HIP_CHECK(hipGetLastError());
+    const int npair = (int)pairs_h.size();
+    launch_pair_norm(psh_d.p, npair, Pn_d.p, nsh, Pcd_d.p, s);
+    HIP_CHECK(hipGetLastError());
     const long long nitems = (long long)items_h.size();
     const double mthr = (mix && allow_mix) ? mix_thr : -1.0;
-    launch_task_count(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, mthr, counts_d.p, s);
+    const ScanArgs sa{items_d.p, nitems, segs_d.p, (int)segs_h.size(), buckets_d.p, pair_bucket_d.p, psh_d.p,
+                      Q_d.p, Pn_d.p, Pcd_d.p, nsh, pmax_d.p, eri_thr, mthr, item_cp_d.p, cp_nk_d.p,
+                      std::abs(lds_max)};
+    launch_task_count(sa, counts_d.p, nslot_d.p, s);
     HIP_CHECK(hipGetLastError());
     launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, 2 * nitems + 1, s);
     HIP_CHECK(hipGetLastError());
+    launch_sum(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, sums_d.p, 2 * nitems, s);
+    HIP_CHECK(hipGetLastError());
+    if (slots) {
+      launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, nslot_d.p, jslot_d.p, nitems + 1, s);
+      HIP_CHECK(hipGetLastError());
+      launch_sum(scan_tmp_d.p, scan_tmp_bytes, nslot_d.p, sums_d.p + 1, nitems, s);
+      HIP_CHECK(hipGetLastError());
+    }
     const int ncp = (int)cp_bra.size();
     HIP_CHECK(hipMemsetAsync(cstat_d.p, 0, sizeof(double) * 4 * ncp, s));
     launch_class_stats(counts_d.p, items_d.p, nitems, item_cp_d.p, buckets_d.p, pairs_d.p, ncp, cstat_d.p,
@@ -629,12 +733,21 @@ struct Engine {
     HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * 2 * (ncp + 1), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&total, offsets_d.p + 2 * nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&bad, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    long long nslots = 0, sums[2] = {0, 0};
+    if (slots) HIP_CHECK(hipMemcpyAsync(&nslots, jslot_d.p + nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(sums, sums_d.p, sizeof(sums), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(cstat_h.data(), cstat_d.p, sizeof(double) * 4 * ncp, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (bad) throw std::runtime_error("Density matrix contains NaN/Inf values.");
+    // the scanned totals size the task list and the share slots: check them
+    // against an independent sum before any kernel writes there
+    if (total != sums[0] || total < 0 || (slots && (nslots != sums[1] || nslots < 0)))
+      throw std::runtime_error("task-list scan totals disagree with their sums");
     if (total > 0) tasks_d.alloc((size_t)total);
-    launch_task_write(items_d.p, nitems, buckets_d.p, pair_bucket_d.p, psh_d.p, Q_d.p, Pn_d.p, nsh,
-                      pmax_d.p, eri_thr, mthr, counts_d.p, offsets_d.p, tasks_d.p, s);
+    jslot_total = nslots;
+    if (slots && nslots > 0) jbuf_d.alloc((size_t)nslots);
+    launch_task_write(sa, counts_d.p, offsets_d.p, tasks_d.p, jslot_d.p, (slots && nslots > 0) ? jbuf_d.p : nullptr,
+                      s);
     HIP_CHECK(hipGetLastError());
     return bval;
   }
@@ -687,7 +800,7 @@ struct Engine {
     const size_t nn = (size_t)nbf * nbf;
     HIP_CHECK(hipEventRecord(ev[0], s));
     const int ncp = (int)cp_bra.size();
-    std::vector<long long> bval = screen(dP, s, true);
+    std::vector<long long> bval = screen(dP, s, true, jcd_slots);
     HIP_CHECK(hipEventRecord(ev[1], s));
     if (dJ) HIP_CHECK(hipMemsetAsync(dJ, 0, nn * sizeof(double), s));
     if (dK) HIP_CHECK(hipMemsetAsync(dK, 0, nn * sizeof(double), s));
@@ -704,28 +817,8 @@ struct Engine {
       Kw = K_d.p;
       HIP_CHECK(hipMemsetAsync(Kw, 0, nn * sizeof(double), s));
     }
-    // J_cd shares per task, summed per ket pair after the class kernels, when
-    // the task list fits the 32-bit sort keys and offsets
-    const long long ntask_all = bval[2 * (ncp + 1) - 1];
-    long long jsize = 0;
-    for (int prec = 0; prec < 2; ++prec) {
-      const long long* b = bval.data() + prec * (ncp + 1);
-      for (int cp = 0; cp < ncp; ++cp) jsize += (b[cp + 1] - b[cp]) * (long long)nfun_pair_class(cp_ket[cp]);
-    }
-    const int npair = (int)pairs_h.size();
-    const bool jsort = jcd_sort && ntask_all > 0 && ntask_all < (1LL << 31) - 1 && jsize < (1LL << 32);
-    int jbits = 1;
-    while ((1 << jbits) < npair) ++jbits;
-    size_t jtmp = 0;
-    if (jsort) {
-      jbuf_d.alloc((size_t)jsize);
-      jkeys_d.alloc(2 * (size_t)ntask_all);
-      jvals_d.alloc(2 * (size_t)ntask_all);
-      jruns_d.alloc(2 * (size_t)npair);
-      jtmp = jcd_sort_temp_bytes(ntask_all, jbits);
-      jsort_tmp_d.alloc(jtmp);
-    }
-    long long jbase = 0;
+    // J_cd shares in the items' slots, summed per ket pair after the class kernels
+    const bool jslots = jcd_slots && jslot_total > 0;
     long long nq = 0, nq32 = 0;
     for (int prec = 0; prec < 2; ++prec) {
       const long long* b = bval.data() + prec * (ncp + 1);
@@ -737,8 +830,7 @@ struct Engine {
                    prec == 1, ppos32_d.p, pk32_d.p, &ra32,
                    segs_d.p + cp_seg_begin[cp], cp_seg_begin[cp + 1] - cp_seg_begin[cp],
                    offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_d.p, lds_max,
-                   jsort ? jbuf_d.p : nullptr, b[cp], jbase, jkeys_d.p, jvals_d.p, seg_min_waves};
-        jbase += n * (long long)nfun_pair_class(cp_ket[cp]);
+                   jslots ? jbuf_d.p : nullptr, jslot_d.p, qpos_d.p, seg_min_waves};
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
         if (e) HIP_CHECK(hipEventRecord(e[0], s));
         launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
@@ -746,11 +838,12 @@ struct Engine {
         if (e) HIP_CHECK(hipEventRecord(e[1], s));
       }
     }
-    if (jsort) {
-      launch_jcd_reduce(jsort_tmp_d.p, jtmp, jkeys_d.p, jkeys_d.p + ntask_all, jvals_d.p, jvals_d.p + ntask_all,
-                        ntask_all, jbits, pairs_d.p, npair, cls_pair_begin.data(), (int)cls_pair_begin.size() - 1,
-                        jruns_d.p, jruns_d.p + npair, jbuf_d.p, Jw, nbf, s);
-      HIP_CHECK(hipGetLastError());
+    if (jslots) {
+      for (const JTileGroup& g : jt_groups) {
+        launch_jcd_tiles(jtiles_d.p + g.begin, g.count, g.nk, g.nsb, nslot_d.p, jslot_d.p, jbuf_d.p, pairs_d.p, Jw,
+                         nbf, s);
+        HIP_CHECK(hipGetLastError());
+      }
     }
     HIP_CHECK(hipEventRecord(ev[2], s));
     launch_symmetrize(dJ, dK, nbf, s);

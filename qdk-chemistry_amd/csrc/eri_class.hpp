@@ -50,26 +50,47 @@ constexpr int seg_bch(int hi) {
 // Accumulators per thread: bra functions x ket cartesians of one chunk.  48
 // for classes up to d shells (their whole ket block fits); classes with an f
 // shell hold fewer and are also chunked over the ket cartesians, so that no
-// kernel instance spills registers.
+// kernel instance spills registers (24, and 16 from L = 11 on).
 constexpr int kAccMax = 48;
-constexpr int kAccMaxF = 16;
+// A class whose kernels spilled with the default is compiled with its own
+// QFE_JK_ACC_F (defined by its generated source).
+template <class C>
+constexpr int jk_acc() {
+  if (C::LA <= 2 && C::LB <= 2 && C::LC <= 2 && C::LD <= 2) return kAccMax;
+#ifdef QFE_JK_ACC_F
+  return QFE_JK_ACC_F;
+#else
+  return C::L >= 11 ? 16 : 24;
+#endif
+}
 template <class C>
 struct JKChunks {
   static constexpr bool F = C::LA > 2 || C::LB > 2 || C::LC > 2 || C::LD > 2;
-  static constexpr int ACC = F ? kAccMaxF : kAccMax;
+  static constexpr int ACC = jk_acc<C>();
   static constexpr int KCH = C::NKETC <= ACC ? C::NKETC : ACC;  // ket cartesians per chunk
   static constexpr int NKCH = (C::NKETC + KCH - 1) / KCH;
   // bra functions per chunk: as many as the accumulators allow, then fewer
   // until no chunk's segment kernel needs more than kSegRowsMax LDS rows
   static constexpr int BCH_ = ACC / KCH < 1 ? 1 : (ACC / KCH > C::NBRA ? C::NBRA : ACC / KCH);
   static constexpr int BCH = seg_bch<C>(BCH_);
-  static constexpr int NBCH = (C::NBRA + BCH - 1) / BCH;
+  static constexpr int NBU = (C::NBRA + BCH - 1) / BCH;  // uniform bra chunks of BCH functions
+  // or row-aligned: each a function's NSB bra functions in MP near-equal parts
+  // of at most kSegRowsMax - 1 (one a row plus its b rows), when that gives
+  // fewer chunks (wide b shells, where uniform chunks would straddle rows)
+  static constexpr int WA = BCH_ < C::NSB ? (BCH_ < kSegRowsMax - 1 ? BCH_ : kSegRowsMax - 1)
+                                          : (C::NSB < kSegRowsMax - 1 ? C::NSB : kSegRowsMax - 1);
+  static constexpr int MP = (C::NSB + WA - 1) / WA;
+  static constexpr bool ALIGNED = C::NSA * MP < NBU;
+  static constexpr int NBCH = ALIGNED ? C::NSA * MP : NBU;
   static constexpr int NCH = NBCH * NKCH;  // chunk c: bra chunk c / NKCH, ket chunk c % NKCH
   template <int CI>
   struct At {
     static constexpr int b = CI / NKCH, k = CI % NKCH;
-    static constexpr int Q0 = b * BCH, Q1 = (Q0 + BCH < C::NBRA) ? Q0 + BCH : C::NBRA;
+    static constexpr int Q0 = ALIGNED ? (b / MP) * C::NSB + (b % MP) * C::NSB / MP : b * BCH;
+    static constexpr int Q1 = ALIGNED ? (b / MP) * C::NSB + (b % MP + 1) * C::NSB / MP
+                                      : ((Q0 + BCH < C::NBRA) ? Q0 + BCH : C::NBRA);
     static constexpr int K0 = k * KCH, K1 = (K0 + KCH < C::NKETC) ? K0 + KCH : C::NKETC;
+    static_assert(seg_rows<C>(Q0, Q1) <= kSegRowsMax, "bra chunk needs too many LDS rows");
   };
 };
 
@@ -112,9 +133,8 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
                                                     const R* __restrict__ pk, RysArgsT<R> ra,
                                                     const double* __restrict__ P, double* __restrict__ J,
                                                     double* __restrict__ K, int nbf, double kscale,
-                                                    double* __restrict__ jbuf, long long jrange0, long long jbase,
-                                                    unsigned* __restrict__ jkeys, unsigned* __restrict__ jvals,
-                                                    int jfirst, int nsplit) {
+                                                    double* __restrict__ jbuf, const long long* __restrict__ jslot,
+                                                    const int2* __restrict__ qpos, int jfirst, int nsplit) {
   using S = SegRows<C, Q0, Q1>;
   extern __shared__ double lds[];
   // block = (segment, part): a long segment may be split over nsplit blocks,
@@ -199,12 +219,12 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
       });
     });
     if (jbuf) {
-      // J_cd share of this task, summed per ket pair after the class kernels
-      double* jt = jbuf + jbase + (size_t)(t - jrange0) * C::NKETS;
+      // J_cd share of this task in its item's slots, summed per ket pair
+      // after the class kernels
+      const int2 qp = qpos[tk.y];
+      double* jt = jbuf + jslot[sg.x + qp.x] + (long long)qp.y * C::NKETS;
       if (jfirst) {
         sfor<C::NKETS>([&](auto f) { jt[f] = jkl[f]; });
-        jkeys[t] = (unsigned)tk.y;
-        jvals[t] = (unsigned)(jbase + (t - jrange0) * C::NKETS);
       } else {
         sfor<C::NKETS>([&](auto f) { jt[f] += jkl[f]; });
       }
@@ -358,11 +378,10 @@ struct ClassOps {
     const dim3 grid((unsigned)((long long)a.nseg * nsplit));
     if constexpr (sizeof(R) == 4)
       hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos32, a.pk32,
-                         *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase, a.jkeys, a.jvals,
-                         jfirst, nsplit);
+                         *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst, nsplit);
     else
       hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos, a.pk,
-                         *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jrange0, a.jbase, a.jkeys, a.jvals, jfirst,
+                         *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst,
                          nsplit);
     return true;
   }

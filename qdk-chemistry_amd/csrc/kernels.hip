@@ -55,55 +55,54 @@ __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __r
 
 // ------------------------------------------------------------ task list
 // work item = (bra pair p, ket bucket b); one wavefront scans bucket b.
-__device__ __forceinline__ bool quartet_passes(int2 P1, double Q1, int2 P2, double Q2,
-                                               const float* __restrict__ Pn, int nsh, double thr) {
-  const size_t a = P1.x, b = P1.y, c = P2.x, d = P2.y;
-  float m = Pn[a * nsh + b];
-  m = fmaxf(m, Pn[c * nsh + d]);
-  m = fmaxf(m, Pn[a * nsh + c]);
-  m = fmaxf(m, Pn[a * nsh + d]);
-  m = fmaxf(m, Pn[b * nsh + c]);
-  m = fmaxf(m, Pn[b * nsh + d]);
-  return !((double)m * Q1 * Q2 < thr);
-}
-
+//
 // counts/offsets are [2][nitems]: the FP64 part, then the FP32 part (quartets
 // whose Schwarz bound Q_ab Q_cd is below mix_thr when mixed precision is on).
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ items, long long nitems,
-                                                   const int2* __restrict__ buckets,
-                                                   const int* __restrict__ pair_bucket,
-                                                   const int2* __restrict__ psh,
-                                                   const double* __restrict__ Q,
-                                                   const float* __restrict__ Pn, int nsh,
-                                                   const double* __restrict__ pmax_ptr, double thr,
-                                                   double mix_thr, int* __restrict__ counts,
-                                                   const long long* __restrict__ offsets,
-                                                   int2* __restrict__ tasks) {
-  const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  const int lane = threadIdx.x & 63;
-  if (item >= nitems) return;
-  if (WRITE && counts[item] == 0 && counts[nitems + item] == 0) return;
+// The ket pairs q of an item with Q_p Q_q >= thr / pmax form a prefix of its
+// bucket range (the bucket is sorted by Q descending); only they can pass the
+// density test.  The count pass also stores nslot[item] = prefix length x nk
+// (nk = functions per ket pair, cp_nk[item_cp[item]]): the item's J_cd share
+// slots.  With jbuf, the write pass stores zero shares in the slots of the
+// prefix quartets that fail the density test, so that every slot of the
+// prefix is written once per build (the others by the class kernels).
+//
+// One wave scans one item.  pn(c, d) returns max(Pn[a][c], Pn[a][d],
+// Pn[b][c], Pn[b][d]) for the bra shells (a, b); Pcd[q] = Pn of ket pair q.
+template <bool WRITE, class PN>
+__device__ __forceinline__ void scan_item(long long item, int lane, const int2* __restrict__ items, long long nitems,
+                                          const int2* __restrict__ buckets, const int* __restrict__ pair_bucket,
+                                          const int2* __restrict__ psh, const double* __restrict__ Q,
+                                          const float* __restrict__ Pcd, float pab, const PN& pn, double cut,
+                                          double thr, double mix_thr, int* __restrict__ counts,
+                                          const long long* __restrict__ offsets, int2* __restrict__ tasks,
+                                          int nk, int* __restrict__ nslot, const long long* __restrict__ jslot,
+                                          double* __restrict__ jbuf) {
+  const bool zero_fill = WRITE && jbuf != nullptr;
+  if (WRITE && !zero_fill && counts[item] == 0 && counts[nitems + item] == 0) return;
   const int2 it = items[item];
   const int p = it.x;
   const int2 bk = buckets[it.y];
   int end = bk.y;
   if (pair_bucket[p] == it.y) end = p + 1;  // same bucket: q <= p
   const double Qp = Q[p];
-  const int2 Pp = psh[p];
-  const double cut = thr / (*pmax_ptr);
   const long long base64 = WRITE ? offsets[item] : 0;
   const long long base32 = WRITE ? offsets[nitems + item] : 0;
-  int c64 = 0, c32 = 0;
+  double* const js = zero_fill ? jbuf + jslot[item] : nullptr;
+  int c64 = 0, c32 = 0, npre = 0;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int q0 = bk.x; q0 < end; q0 += 64) {
     // sorted by Q descending: if the first ket fails the Schwarz bound, all later do
     if (Qp * Q[q0] < cut) break;
     const int q = q0 + lane;
-    bool ok = false, lo = false;
+    bool ok = false, lo = false, pre = false;
     if (q < end) {
       const double Qq = Q[q];
-      ok = quartet_passes(Pp, Qp, psh[q], Qq, Pn, nsh, thr);
+      pre = !(Qp * Qq < cut);
+      if (pre) {
+        const int2 cd = psh[q];
+        const float m = fmaxf(fmaxf(pab, Pcd[q]), pn(cd.x, cd.y));
+        ok = !((double)m * Qp * Qq < thr);  // libint2_direct.cpp:400-402
+      }
       lo = Qp * Qq < mix_thr;
     }
     const unsigned long long m64 = __ballot(ok && !lo);
@@ -114,13 +113,85 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
       else
         tasks[base64 + c64 + __popcll(m64 & below)] = make_int2(p, q);
     }
+    if (zero_fill && pre && !ok)
+      for (int f = 0; f < nk; ++f) js[(size_t)(q - bk.x) * nk + f] = 0.0;
     c64 += __popcll(m64);
     c32 += __popcll(m32);
+    npre += __popcll(__ballot(pre));
   }
   if (!WRITE && lane == 0) {
     counts[item] = c64;
     counts[nitems + item] = c32;
+    nslot[item] = npre * nk;
   }
+}
+
+// One block per segment (the items of one bra pair (a, b) in one class
+// pair): the density-norm rows of a and b are staged in LDS, so the four
+// bra-ket norms of a candidate are LDS reads and only Pcd[q] comes from HBM
+// (coalesced, like Q and psh).  Waves take the segment's items in turn.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_task_scan_seg(const int2* __restrict__ segs, const int2* __restrict__ items,
+                                                       long long nitems, const int2* __restrict__ buckets,
+                                                       const int* __restrict__ pair_bucket,
+                                                       const int2* __restrict__ psh, const double* __restrict__ Q,
+                                                       const float* __restrict__ Pn, const float* __restrict__ Pcd,
+                                                       int nsh, const double* __restrict__ pmax_ptr, double thr,
+                                                       double mix_thr, int* __restrict__ counts,
+                                                       const long long* __restrict__ offsets,
+                                                       int2* __restrict__ tasks, const int* __restrict__ item_cp,
+                                                       const int* __restrict__ cp_nk, int* __restrict__ nslot,
+                                                       const long long* __restrict__ jslot,
+                                                       double* __restrict__ jbuf) {
+  extern __shared__ float prow[];  // [2][nsh]: rows a and b of Pn
+  const int2 sg = segs[blockIdx.x];
+  const int2 ab = psh[items[sg.x].x];
+  for (int c = threadIdx.x; c < nsh; c += blockDim.x) {
+    prow[c] = Pn[(size_t)ab.x * nsh + c];
+    prow[nsh + c] = Pn[(size_t)ab.y * nsh + c];
+  }
+  __syncthreads();
+  const float pab = prow[ab.y];
+  const double cut = thr / (*pmax_ptr);
+  const int nk = cp_nk[item_cp[sg.x]];
+  const int lane = (int)(threadIdx.x & 63);
+  auto pn = [&](int c, int d) { return fmaxf(fmaxf(prow[c], prow[d]), fmaxf(prow[nsh + c], prow[nsh + d])); };
+  for (int item = sg.x + (int)(threadIdx.x >> 6); item < sg.y; item += (int)(blockDim.x >> 6))
+    scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, pab, pn, cut, thr, mix_thr,
+                     counts, offsets, tasks, nk, nslot, jslot, jbuf);
+}
+
+// The same scan with one wave per item and the norms gathered from HBM (for
+// shell counts whose two rows do not fit the LDS).
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ items, long long nitems,
+                                                   const int2* __restrict__ buckets,
+                                                   const int* __restrict__ pair_bucket,
+                                                   const int2* __restrict__ psh, const double* __restrict__ Q,
+                                                   const float* __restrict__ Pn, const float* __restrict__ Pcd,
+                                                   int nsh, const double* __restrict__ pmax_ptr, double thr,
+                                                   double mix_thr, int* __restrict__ counts,
+                                                   const long long* __restrict__ offsets,
+                                                   int2* __restrict__ tasks, const int* __restrict__ item_cp,
+                                                   const int* __restrict__ cp_nk, int* __restrict__ nslot,
+                                                   const long long* __restrict__ jslot, double* __restrict__ jbuf) {
+  const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  if (item >= nitems) return;
+  const int2 ab = psh[items[item].x];
+  const float* ra = Pn + (size_t)ab.x * nsh;
+  const float* rb = Pn + (size_t)ab.y * nsh;
+  auto pn = [&](int c, int d) { return fmaxf(fmaxf(ra[c], ra[d]), fmaxf(rb[c], rb[d])); };
+  scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, ra[ab.y], pn,
+                   thr / (*pmax_ptr), thr, mix_thr, counts, offsets, tasks, cp_nk[item_cp[item]], nslot, jslot,
+                   jbuf);
+}
+
+// Pcd[q] = Pn of the shell pair of pair q
+__global__ void k_pair_norm(const int2* __restrict__ psh, int npair, const float* __restrict__ Pn, int nsh,
+                            float* __restrict__ Pcd) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < npair) Pcd[q] = Pn[(size_t)psh[q].x * nsh + psh[q].y];
 }
 
 // per class pair: surviving shell quartets and their primitive quartets
@@ -191,37 +262,62 @@ void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* s
   hipLaunchKernelGGL(k_max_reduce, dim3(g), dim3(256), 0, s, Pnd, n, pmax);
 }
 
-void launch_task_count(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
-                       const int2* pairs, const double* Q, const float* Pn, int nsh,
-                       const double* pmax, double thr, double mix_thr, int* counts, hipStream_t s) {
-  if (nitems <= 0) return;
-  const long long threads = nitems * 64;
-  const long long grid = (threads + 255) / 256;
-  hipLaunchKernelGGL((k_task_scan<false>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
-                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, mix_thr, counts, (const long long*)nullptr,
-                     (int2*)nullptr);
+void launch_pair_norm(const int2* psh, int npair, const float* Pn, int nsh, float* Pcd, hipStream_t s) {
+  if (npair <= 0) return;
+  hipLaunchKernelGGL(k_pair_norm, dim3((npair + 255) / 256), dim3(256), 0, s, psh, npair, Pn, nsh, Pcd);
 }
 
-void launch_task_write(const int2* items, long long nitems, const int2* buckets, const int* pair_bucket,
-                       const int2* pairs, const double* Q, const float* Pn, int nsh,
-                       const double* pmax, double thr, double mix_thr, const int* counts,
-                       const long long* offsets, int2* tasks, hipStream_t s) {
-  if (nitems <= 0) return;
-  const long long threads = nitems * 64;
-  const long long grid = (threads + 255) / 256;
-  hipLaunchKernelGGL((k_task_scan<true>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems, buckets,
-                     pair_bucket, pairs, Q, Pn, nsh, pmax, thr, mix_thr, (int*)counts, offsets, tasks);
+template <bool WRITE>
+static void task_scan(const ScanArgs& a, int* counts, const long long* offsets, int2* tasks, int* nslot,
+                      const long long* jslot, double* jbuf, hipStream_t s) {
+  if (a.nitems <= 0) return;
+  const size_t lds = 2 * sizeof(float) * (size_t)a.nsh;
+  if (a.segs && a.nseg > 0 && lds <= (size_t)a.lds_max) {
+    auto kern = k_task_scan_seg<WRITE>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(256), lds, s, a.segs, a.items, a.nitems, a.buckets,
+                       a.pair_bucket, a.psh, a.Q, a.Pn, a.Pcd, a.nsh, a.pmax, a.thr, a.mix_thr, counts, offsets,
+                       tasks, a.item_cp, a.cp_nk, nslot, jslot, jbuf);
+  } else {
+    const long long threads = a.nitems * 64;
+    hipLaunchKernelGGL((k_task_scan<WRITE>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, a.items,
+                       a.nitems, a.buckets, a.pair_bucket, a.psh, a.Q, a.Pn, a.Pcd, a.nsh, a.pmax, a.thr, a.mix_thr,
+                       counts, offsets, tasks, a.item_cp, a.cp_nk, nslot, jslot, jbuf);
+  }
 }
+
+void launch_task_count(const ScanArgs& a, int* counts, int* nslot, hipStream_t s) {
+  task_scan<false>(a, counts, nullptr, nullptr, nslot, nullptr, nullptr, s);
+}
+
+void launch_task_write(const ScanArgs& a, const int* counts, const long long* offsets, int2* tasks,
+                       const long long* jslot, double* jbuf, hipStream_t s) {
+  task_scan<true>(a, (int*)counts, offsets, tasks, nullptr, jslot, jbuf, s);
+}
+
+// The scans and sums read the int counts widened to 64 bits: hipcub
+// accumulates in the input's value type, and the J_cd slot total of a large
+// system exceeds 2^31.
+struct WidenInt {
+  __host__ __device__ long long operator()(int x) const { return (long long)x; }
+};
+using WideInts = rocprim::transform_iterator<const int*, WidenInt, long long>;
 
 size_t scan_temp_bytes(long long n) {
-  size_t bytes = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int*)nullptr, (long long*)nullptr, (int)n);
-  return bytes;
+  size_t a = 0, b = 0;
+  const WideInts it(nullptr, WidenInt());
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, a, it, (long long*)nullptr, (int)n);
+  (void)hipcub::DeviceReduce::Sum(nullptr, b, it, (long long*)nullptr, (int)n);
+  return a > b ? a : b;
 }
 
 void launch_exclusive_scan(void* temp, size_t temp_bytes, const int* in, long long* out, long long n,
                            hipStream_t s) {
-  (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int)n, s);
+  (void)hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, WideInts(in, WidenInt()), out, (int)n, s);
+}
+
+void launch_sum(void* temp, size_t temp_bytes, const int* in, long long* out, long long n, hipStream_t s) {
+  (void)hipcub::DeviceReduce::Sum(temp, temp_bytes, WideInts(in, WidenInt()), out, (int)n, s);
 }
 
 void launch_gather_offsets(const long long* offsets, const long long* idx, int n, long long* out,
@@ -243,103 +339,100 @@ void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
 }
 
 // ------------------------------------------------------------ J_cd by ket pair
-__global__ void k_run_bounds(const unsigned* __restrict__ keys, long long n, int* __restrict__ run_start,
-                             int* __restrict__ run_end) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const unsigned k = keys[i];
-  if (i == 0 || keys[i - 1] != k) run_start[k] = (int)i;
-  if (i == n - 1 || keys[i + 1] != k) run_end[k] = (int)(i + 1);
-}
-
-__device__ __forceinline__ double wave_sum64(double v) {
-  v += __shfl_xor(v, 32);
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 8);
-  v += __shfl_xor(v, 4);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 1);
-  return v;
-}
-
-// one wave per ket pair of one pair class (NK functions per pair): each lane
-// adds whole shares of the run (NK contiguous doubles), then the wave sums
-// them and adds one atomic per J element
+// Share slots: item (p, kb) owns nslot[item] / NK positions j (ket pair
+// q = first pair of bucket kb + j), NK doubles each, from jbuf + jslot[item].
+// A tile is up to 256 consecutive bra pairs of one bra bucket (items
+// item0 + i * stride, i < np) against positions [j0, j0 + kJcdTileKet) of one
+// ket bucket whose first pair is q0.  One wave per tile: each lane owns one
+// position of a 64-wide column block, walks the tile's items (their prefix
+// lengths and slot bases are loaded once, four per lane, and broadcast), reads
+// its share with coalesced loads, and adds the column sums to J.
 template <int NK>
-__global__ __launch_bounds__(256) void k_jcd_sum(const PairHdr* __restrict__ pairs, int q0, int q1, int nsb,
-                                                 const int* __restrict__ run_start, const int* __restrict__ run_end,
-                                                 const unsigned* __restrict__ vals, const double* __restrict__ jbuf,
-                                                 double* __restrict__ J, int nbf) {
-  const int q = q0 + (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+__global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tiles, int ntiles,
+                                                   const int* __restrict__ nslot,
+                                                   const long long* __restrict__ jslot,
+                                                   const double* __restrict__ jbuf,
+                                                   const PairHdr* __restrict__ pairs, int nsb,
+                                                   double* __restrict__ J, int nbf) {
+  const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int lane = (int)(threadIdx.x & 63);
-  if (q >= q1) return;  // whole waves
-  const int i0 = run_start[q], i1 = run_end[q];
-  if (i0 >= i1) return;
-  double acc[NK];
+  if (w >= ntiles) return;  // whole waves
+  const int4 tl = tiles[w];
+  const int stride = tl.y >> 16, np = tl.y & 0xffff;
+  int nl[4];
+  long long bl[4];
+  int nmax = 0;
 #pragma unroll
-  for (int f = 0; f < NK; ++f) acc[f] = 0.0;
-  for (int i = i0 + lane; i < i1; i += 64) {
-    const double* sh = jbuf + vals[i];
-#pragma unroll
-    for (int f = 0; f < NK; ++f) acc[f] += sh[f];
+  for (int k = 0; k < 4; ++k) {
+    const int i = lane + 64 * k;
+    nl[k] = 0;
+    bl[k] = 0;
+    if (i < np) {
+      const int item = tl.x + i * stride;
+      nl[k] = nslot[item] / NK;
+      bl[k] = jslot[item];
+    }
+    nmax = nl[k] > nmax ? nl[k] : nmax;
   }
-  const PairHdr h = pairs[q];
 #pragma unroll
-  for (int f = 0; f < NK; ++f) {
-    const double v = wave_sum64(acc[f]);
-    if (lane == 0 && v != 0.0) unsafeAtomicAdd(&J[(size_t)(h.bfa + f / nsb) * nbf + h.bfb + f % nsb], v);
+  for (int o = 32; o > 0; o >>= 1) {
+    const int v = __shfl_xor(nmax, o);
+    nmax = v > nmax ? v : nmax;
+  }
+  const int jend = tl.w + kJcdTileKet < nmax ? tl.w + kJcdTileKet : nmax;
+  for (int jc = tl.w; jc < jend; jc += 64) {
+    const int j = jc + lane;
+    double acc[NK];
+#pragma unroll
+    for (int f = 0; f < NK; ++f) acc[f] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int lim = np - 64 * k < 64 ? np - 64 * k : 64;
+      for (int l = 0; l < lim; ++l) {
+        const int n = __builtin_amdgcn_readlane(nl[k], l);
+        if (n <= jc) continue;  // the same for the whole wave
+        const int lo = __builtin_amdgcn_readlane((int)(bl[k] & 0xffffffffLL), l);
+        const int hi = __builtin_amdgcn_readlane((int)(bl[k] >> 32), l);
+        const long long b = ((long long)hi << 32) | (unsigned)lo;
+        if (j < n) {
+          const double* sh = jbuf + b + (long long)j * NK;
+#pragma unroll
+          for (int f = 0; f < NK; ++f) acc[f] += sh[f];
+        }
+      }
+    }
+    if (j < nmax) {
+      const PairHdr h = pairs[tl.z + j];
+#pragma unroll
+      for (int f = 0; f < NK; ++f)
+        if (acc[f] != 0.0) unsafeAtomicAdd(&J[(size_t)(h.bfa + f / nsb) * nbf + h.bfb + f % nsb], acc[f]);
+    }
   }
 }
 
 template <int NK>
-static void jcd_sum_class(const PairHdr* pairs, int q0, int q1, int nsb, const int* run_start, const int* run_end,
-                          const unsigned* vals, const double* jbuf, double* J, int nbf, hipStream_t s) {
-  const long long threads = (long long)(q1 - q0) * 64;
-  hipLaunchKernelGGL((k_jcd_sum<NK>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, pairs, q0, q1, nsb,
-                     run_start, run_end, vals, jbuf, J, nbf);
+static void jcd_tiles_launch(const int4* tiles, int ntiles, const int* nslot, const long long* jslot,
+                             const double* jbuf, const PairHdr* pairs, int nsb, double* J, int nbf, hipStream_t s) {
+  const long long threads = (long long)ntiles * 64;
+  hipLaunchKernelGGL((k_jcd_tiles<NK>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tiles, ntiles,
+                     nslot, jslot, jbuf, pairs, nsb, J, nbf);
 }
 
-size_t jcd_sort_temp_bytes(long long n, int bits) {
-  size_t bytes = 0;
-  hipcub::DoubleBuffer<unsigned> k(nullptr, nullptr), v(nullptr, nullptr);
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)n, 0, bits);
-  return bytes;
-}
-
-void launch_jcd_reduce(void* temp, size_t temp_bytes, unsigned* keys0, unsigned* keys1, unsigned* vals0,
-                       unsigned* vals1, long long n, int bits, const PairHdr* pairs, int npair,
-                       const int* cls_begin, int ncls, int* run_start, int* run_end, const double* jbuf, double* J,
-                       int nbf, hipStream_t s) {
-  if (n <= 0) return;
-  hipcub::DoubleBuffer<unsigned> k(keys0, keys1), v(vals0, vals1);
-  (void)hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, k, v, (int)n, 0, bits, s);
-  (void)hipMemsetAsync(run_start, 0, sizeof(int) * npair, s);
-  (void)hipMemsetAsync(run_end, 0, sizeof(int) * npair, s);
-  hipLaunchKernelGGL(k_run_bounds, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, k.Current(), n, run_start,
-                     run_end);
-  // pairs are sorted by class: one launch per pair class, templated on its functions
-  for (int c = 0; c < ncls; ++c) {
-    const int q0 = cls_begin[c], q1 = cls_begin[c + 1];
-    if (q0 >= q1) continue;
-    int la = 0;
-    while ((la + 1) * (la + 2) / 2 <= c) ++la;
-    const int lb = c - la * (la + 1) / 2;
-    const int nsa = la >= 2 ? 2 * la + 1 : (la + 1) * (la + 2) / 2;
-    const int nsb = lb >= 2 ? 2 * lb + 1 : (lb + 1) * (lb + 2) / 2;
-    const unsigned* vs = v.Current();
-    switch (nsa * nsb) {
-      case 1: jcd_sum_class<1>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 3: jcd_sum_class<3>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 5: jcd_sum_class<5>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 7: jcd_sum_class<7>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 9: jcd_sum_class<9>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 15: jcd_sum_class<15>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 21: jcd_sum_class<21>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 25: jcd_sum_class<25>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 35: jcd_sum_class<35>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      case 49: jcd_sum_class<49>(pairs, q0, q1, nsb, run_start, run_end, vs, jbuf, J, nbf, s); break;
-      default: break;  // no other pure pair sizes up to f
-    }
+void launch_jcd_tiles(const int4* tiles, int ntiles, int nk, int nsb, const int* nslot, const long long* jslot,
+                      const double* jbuf, const PairHdr* pairs, double* J, int nbf, hipStream_t s) {
+  if (ntiles <= 0) return;
+  switch (nk) {
+    case 1: jcd_tiles_launch<1>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 3: jcd_tiles_launch<3>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 5: jcd_tiles_launch<5>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 7: jcd_tiles_launch<7>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 9: jcd_tiles_launch<9>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 15: jcd_tiles_launch<15>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 21: jcd_tiles_launch<21>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 25: jcd_tiles_launch<25>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 35: jcd_tiles_launch<35>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 49: jcd_tiles_launch<49>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    default: break;  // no other pure pair sizes up to f
   }
 }
 

@@ -40,13 +40,13 @@ struct JKLaunch {
   const int2* tasks_all;
   int lds_max;        // dynamic LDS bytes a block may request (negative: exchange rows only)
   // J_cd by ket pair (jbuf null: one global atomic per task function pair):
-  // task t of this launch writes its share to jbuf[jbase + (t - jrange0) * NKETS
-  // + f]; the first chunk of the class stores it with jkeys[t] = ket pair and
-  // jvals[t] = the offset, later chunks add to it
+  // the task (p, q) of segment item range [x, y) writes its share to
+  // jbuf + jslot[x + qpos[q].x] + qpos[q].y * NKETS (qpos = ket bucket of q
+  // relative to the first bucket of its class, position of q in that
+  // bucket); the first chunk of the class stores it, later chunks add to it
   double* jbuf;
-  long long jrange0, jbase;
-  unsigned* jkeys;
-  unsigned* jvals;
+  const long long* jslot;
+  const int2* qpos;
   // keep the density rows in LDS when that still leaves at least this many
   // resident waves per CU (else the K-row-only variant, if it fits more)
   int seg_min_waves;

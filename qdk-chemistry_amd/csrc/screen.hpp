@@ -12,17 +12,44 @@ struct PairHdr;
 
 void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
                        float* Pn, double* Pnd, int* bad, double* pmax, hipStream_t s);
-void launch_task_count(const int2* items, long long nitems, const int2* buckets,
-                       const int* pair_bucket, const int2* pair_shells, const double* Q,
-                       const float* Pn, int nsh, const double* pmax, double thr, double mix_thr,
-                       int* counts, hipStream_t s);
-void launch_task_write(const int2* items, long long nitems, const int2* buckets,
-                       const int* pair_bucket, const int2* pair_shells, const double* Q,
-                       const float* Pn, int nsh, const double* pmax, double thr, double mix_thr,
-                       const int* counts, const long long* offsets, int2* tasks, hipStream_t s);
+// Inputs of the task-list scan: work items (bra pair, ket bucket) and their
+// segments (the items of one bra pair within one class pair), buckets, pair
+// shells and Schwarz factors, the density norms Pn (nsh x nsh) and per pair
+// (Pcd), the thresholds, and per item the class pair and per class pair the
+// functions per ket pair.  With segments and 8 nsh bytes of LDS available
+// (lds_max) the scan runs one block per segment, else one wave per item.
+struct ScanArgs {
+  const int2* items;
+  long long nitems;
+  const int2* segs;
+  int nseg;
+  const int2* buckets;
+  const int* pair_bucket;
+  const int2* psh;
+  const double* Q;
+  const float* Pn;
+  const float* Pcd;
+  int nsh;
+  const double* pmax;
+  double thr, mix_thr;
+  const int* item_cp;
+  const int* cp_nk;
+  int lds_max;
+};
+void launch_pair_norm(const int2* psh, int npair, const float* Pn, int nsh, float* Pcd, hipStream_t s);
+// counts [2][nitems] (FP64, FP32 survivors) and nslot[nitems] (J_cd slots)
+void launch_task_count(const ScanArgs& a, int* counts, int* nslot, hipStream_t s);
+// tasks at the scanned offsets; with jbuf, zero J_cd shares for the prefix
+// quartets that fail the density test (slot bases jslot)
+void launch_task_write(const ScanArgs& a, const int* counts, const long long* offsets, int2* tasks,
+                       const long long* jslot, double* jbuf, hipStream_t s);
+// workspace of launch_exclusive_scan and launch_sum for up to n ints (both
+// accumulate in 64 bits)
 size_t scan_temp_bytes(long long n);
 void launch_exclusive_scan(void* temp, size_t temp_bytes, const int* in, long long* out,
                            long long n, hipStream_t s);
+// 64-bit sum of n ints (the check of a scan's total); same workspace
+void launch_sum(void* temp, size_t temp_bytes, const int* in, long long* out, long long n, hipStream_t s);
 void launch_gather_offsets(const long long* offsets, const long long* idx, int n, long long* out,
                            hipStream_t s);
 void launch_class_stats(const int* counts, const int2* items, long long nitems, const int* item_cp,
@@ -30,20 +57,18 @@ void launch_class_stats(const int* counts, const int2* items, long long nitems, 
                         hipStream_t s);
 void launch_symmetrize(double* J, double* K, int n, hipStream_t s);
 
-// J_cd by ket pair: the class kernels write each task's J_cd share to jbuf
-// (NKETS doubles at a per-task offset) with key = ket pair and value = that
-// offset; the keys are sorted and every ket pair's run is summed into J with
-// one atomic per J element.  jcd_sort_temp_bytes sizes the radix-sort
-// workspace for n keys of `bits` bits; launch_jcd_reduce sorts (double
-// buffers keys0/keys1, vals0/vals1), finds each pair's run (run_start,
-// run_end: npair ints, cleared here) and adds the sums to J.
-size_t jcd_sort_temp_bytes(long long n, int bits);
-// cls_begin (host, ncls + 1 entries): first pair of each pair class (pairs
-// are sorted by class)
-void launch_jcd_reduce(void* temp, size_t temp_bytes, unsigned* keys0, unsigned* keys1, unsigned* vals0,
-                       unsigned* vals1, long long n, int bits, const PairHdr* pairs, int npair,
-                       const int* cls_begin, int ncls, int* run_start, int* run_end, const double* jbuf, double* J,
-                       int nbf, hipStream_t s);
+// J_cd by ket pair.  Item i (bra pair p, ket bucket kb) owns nslot[i] / nk
+// share slots of nk doubles from jbuf + jslot[i], one per position j of the
+// Schwarz prefix of its bucket range (ket pair = first pair of kb + j).  The
+// class kernels store the shares of the surviving quartets, the task-write
+// pass zeros the others.  launch_jcd_tiles sums them per ket pair into J for
+// the tiles of one ket pair size nk (nsb = functions of the pair's second
+// shell); a tile is int4 {first item, stride << 16 | bra pairs (<= 256),
+// first pair of the ket bucket, first position}.
+void launch_jcd_tiles(const int4* tiles, int ntiles, int nk, int nsb, const int* nslot, const long long* jslot,
+                      const double* jbuf, const PairHdr* pairs, double* J, int nbf, hipStream_t s);
+constexpr int kJcdTileBra = 256;   // bra pairs per tile
+constexpr int kJcdTileKet = 1024;  // ket positions per tile
 
 // plain device-to-device copy of n 8-byte words (also the reference kernel for
 // HBM traffic counters: it moves exactly 8n bytes each way)

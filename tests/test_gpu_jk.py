@@ -92,9 +92,10 @@ def test_segment_digestion_variants_agree(monkeypatch, mix):
 
 
 @pytest.mark.parametrize("mix", [False, True])
-def test_jcd_sorted_and_atomic_paths_agree(monkeypatch, mix):
-    """J_cd summed per ket pair after a sort (default) and added per task with
-    global atomics (QFE_JCD_SORT=0) give the same J; both match the oracle."""
+def test_jcd_slots_and_atomic_paths_agree(monkeypatch, mix):
+    """J_cd summed per ket pair from the share slots (default) and added per
+    task with global atomics (QFE_JCD_SLOTS=0) give the same J; both match
+    the oracle."""
     from oracle.oracle import OracleERI
     from qdk_chemistry_amd.basis import load_basis
     from qdk_chemistry_amd.eri import ERIConfig, create
@@ -104,9 +105,9 @@ def test_jcd_sorted_and_atomic_paths_agree(monkeypatch, mix):
     P = random_density(b.nbf, 8, 0.3)
     cfg = ERIConfig(is_mix_precision=mix)
     J1, K1 = create(b, mol, cfg).build_JK(P)
-    monkeypatch.setenv("QFE_JCD_SORT", "0")
+    monkeypatch.setenv("QFE_JCD_SLOTS", "0")
     J0, K0 = create(b, mol, cfg).build_JK(P)
-    print("sorted vs atomic J_cd: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
+    print("slots vs atomic J_cd: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
           flush=True)
     assert np.max(np.abs(J1 - J0)) < 1e-11
     assert np.max(np.abs(K1 - K0)) < 1e-11
@@ -114,6 +115,38 @@ def test_jcd_sorted_and_atomic_paths_agree(monkeypatch, mix):
         Jr, Kr = OracleERI(b, mol).build_JK(P)
         assert np.max(np.abs(J1 - Jr)) < 1e-10
         assert np.max(np.abs(K1 - Kr)) < 1e-10
+
+
+def test_jcd_slots_zeroed_between_builds(monkeypatch):
+    """Share slots of quartets that pass the Schwarz bound but fail the
+    density test are zeroed each build: after a dense density, a
+    block-diagonal one (no density between molecules, so many such quartets)
+    gives the same J as the atomic path and the oracle."""
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(8, 29)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    dense = random_density(b.nbf, 12, 0.3)
+    blk = np.zeros_like(dense)
+    atom_bf = np.concatenate([[sh.atom] * b.nfun(sh.l) for sh in b.shells])  # functions follow the shells
+    mol_bf = atom_bf // 3  # water_cluster lists each molecule's three atoms together
+    for m in range(mol.natoms // 3):
+        i = np.where(mol_bf == m)[0]
+        blk[np.ix_(i, i)] = dense[np.ix_(i, i)]
+    eng = create(b, mol)
+    eng.build_JK(dense)
+    J1, K1 = eng.build_JK(blk)
+    st = eng.stats()
+    monkeypatch.setenv("QFE_JCD_SLOTS", "0")
+    J0, K0 = create(b, mol).build_JK(blk)
+    Jr, Kr = OracleERI(b, mol).build_JK(blk)
+    print("block-diagonal after dense: quartets %d, max|J - J_atomic| %.2e, max|J - J_oracle| %.2e"
+          % (st["quartets"], np.max(np.abs(J1 - J0)), np.max(np.abs(J1 - Jr))), flush=True)
+    assert np.max(np.abs(J1 - J0)) < 1e-11
+    assert np.max(np.abs(J1 - Jr)) < 1e-10
+    assert np.max(np.abs(K1 - Kr)) < 1e-10
 
 
 def test_jk_repeatable():
