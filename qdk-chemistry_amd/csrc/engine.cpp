@@ -213,8 +213,9 @@ struct Engine {
   bool jcd_slots = true;
   int seg_min_waves = 8;
   std::vector<int> cls_pair_begin;  // first pair of each pair class (pairs are class-sorted)
-  DevBuf<int> nslot_d;
-  DevBuf<long long> jslot_d;
+  DevBuf<int> nslot_d, nmask_d;
+  DevBuf<long long> jslot_d, mbase_d;
+  DevBuf<unsigned long long> jmask_d;  // survivor-mask words of every item's Schwarz prefix
   DevBuf<long long> sums_d;  // 64-bit sums of the task counts and of the slot counts
   DevBuf<double> jbuf_d;
   DevBuf<int2> qpos_d;
@@ -544,7 +545,10 @@ struct Engine {
     scan_tmp_d.alloc(scan_tmp_bytes);
     nslot_d.alloc(nitems + 1);
     HIP_CHECK(hipMemsetAsync(nslot_d.p, 0, sizeof(int) * (nitems + 1), stream));
+    nmask_d.alloc(nitems + 1);
+    HIP_CHECK(hipMemsetAsync(nmask_d.p, 0, sizeof(int) * (nitems + 1), stream));
     jslot_d.alloc(nitems + 1);
+    mbase_d.alloc(nitems + 1);
     sums_d.alloc(2);
     HIP_CHECK(hipMemsetAsync(sums_d.p, 0, sizeof(long long) * 2, stream));
     build_jcd_tiles();
@@ -707,7 +711,7 @@ struct Engine {
     const ScanArgs sa{items_d.p, nitems, segs_d.p, (int)segs_h.size(), buckets_d.p, pair_bucket_d.p, psh_d.p,
                       Q_d.p, Pn_d.p, Pcd_d.p, nsh, pmax_d.p, eri_thr, mthr, item_cp_d.p, cp_nk_d.p,
                       std::abs(lds_max)};
-    launch_task_count(sa, counts_d.p, nslot_d.p, s);
+    launch_task_count(sa, counts_d.p, nslot_d.p, nmask_d.p, s);
     HIP_CHECK(hipGetLastError());
     launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, counts_d.p, offsets_d.p, 2 * nitems + 1, s);
     HIP_CHECK(hipGetLastError());
@@ -715,6 +719,8 @@ struct Engine {
     HIP_CHECK(hipGetLastError());
     if (slots) {
       launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, nslot_d.p, jslot_d.p, nitems + 1, s);
+      HIP_CHECK(hipGetLastError());
+      launch_exclusive_scan(scan_tmp_d.p, scan_tmp_bytes, nmask_d.p, mbase_d.p, nitems + 1, s);
       HIP_CHECK(hipGetLastError());
       launch_sum(scan_tmp_d.p, scan_tmp_bytes, nslot_d.p, sums_d.p + 1, nitems, s);
       HIP_CHECK(hipGetLastError());
@@ -733,21 +739,27 @@ struct Engine {
     HIP_CHECK(hipMemcpyAsync(bval.data(), bval_d.p, sizeof(long long) * 2 * (ncp + 1), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&total, offsets_d.p + 2 * nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&bad, bad_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    long long nslots = 0, sums[2] = {0, 0};
-    if (slots) HIP_CHECK(hipMemcpyAsync(&nslots, jslot_d.p + nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
+    long long nslots = 0, nwords = 0, sums[2] = {0, 0};
+    if (slots) {
+      HIP_CHECK(hipMemcpyAsync(&nslots, jslot_d.p + nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipMemcpyAsync(&nwords, mbase_d.p + nitems, sizeof(long long), hipMemcpyDeviceToHost, s));
+    }
     HIP_CHECK(hipMemcpyAsync(sums, sums_d.p, sizeof(sums), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(cstat_h.data(), cstat_d.p, sizeof(double) * 4 * ncp, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (bad) throw std::runtime_error("Density matrix contains NaN/Inf values.");
     // the scanned totals size the task list and the share slots: check them
     // against an independent sum before any kernel writes there
-    if (total != sums[0] || total < 0 || (slots && (nslots != sums[1] || nslots < 0)))
+    if (total != sums[0] || total < 0 || (slots && (nslots != sums[1] || nslots < 0 || nwords < 0)))
       throw std::runtime_error("task-list scan totals disagree with their sums");
     if (total > 0) tasks_d.alloc((size_t)total);
     jslot_total = nslots;
-    if (slots && nslots > 0) jbuf_d.alloc((size_t)nslots);
-    launch_task_write(sa, counts_d.p, offsets_d.p, tasks_d.p, jslot_d.p, (slots && nslots > 0) ? jbuf_d.p : nullptr,
-                      s);
+    if (slots && nslots > 0) {
+      jbuf_d.alloc((size_t)nslots);
+      jmask_d.alloc((size_t)nwords);
+    }
+    launch_task_write(sa, counts_d.p, offsets_d.p, tasks_d.p, mbase_d.p,
+                      (slots && nslots > 0) ? jmask_d.p : nullptr, s);
     HIP_CHECK(hipGetLastError());
     return bval;
   }
@@ -840,8 +852,8 @@ struct Engine {
     }
     if (jslots) {
       for (const JTileGroup& g : jt_groups) {
-        launch_jcd_tiles(jtiles_d.p + g.begin, g.count, g.nk, g.nsb, nslot_d.p, jslot_d.p, jbuf_d.p, pairs_d.p, Jw,
-                         nbf, s);
+        launch_jcd_tiles(jtiles_d.p + g.begin, g.count, g.nk, g.nsb, nslot_d.p, jslot_d.p, mbase_d.p, jmask_d.p,
+                         jbuf_d.p, pairs_d.p, Jw, nbf, s);
         HIP_CHECK(hipGetLastError());
       }
     }

@@ -62,9 +62,10 @@ __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __r
 // bucket range (the bucket is sorted by Q descending); only they can pass the
 // density test.  The count pass also stores nslot[item] = prefix length x nk
 // (nk = functions per ket pair, cp_nk[item_cp[item]]): the item's J_cd share
-// slots.  With jbuf, the write pass stores zero shares in the slots of the
-// prefix quartets that fail the density test, so that every slot of the
-// prefix is written once per build (the others by the class kernels).
+// slots, and nmask[item] = 64-bit words of its survivor mask (one per 64
+// prefix positions).  With jmask, the write pass stores those words (bit j
+// of word k: position 64 k + j survived), so the J_cd reduction reads only
+// the slots that the class kernels wrote.
 //
 // One wave scans one item.  pn(c, d) returns max(Pn[a][c], Pn[a][d],
 // Pn[b][c], Pn[b][d]) for the bra shells (a, b); Pcd[q] = Pn of ket pair q.
@@ -75,10 +76,11 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
                                           const float* __restrict__ Pcd, float pab, const PN& pn, double cut,
                                           double thr, double mix_thr, int* __restrict__ counts,
                                           const long long* __restrict__ offsets, int2* __restrict__ tasks,
-                                          int nk, int* __restrict__ nslot, const long long* __restrict__ jslot,
-                                          double* __restrict__ jbuf) {
-  const bool zero_fill = WRITE && jbuf != nullptr;
-  if (WRITE && !zero_fill && counts[item] == 0 && counts[nitems + item] == 0) return;
+                                          int nk, int* __restrict__ nslot, int* __restrict__ nmask,
+                                          const long long* __restrict__ mbase,
+                                          unsigned long long* __restrict__ jmask) {
+  const bool masks = WRITE && jmask != nullptr;
+  if (WRITE && !masks && counts[item] == 0 && counts[nitems + item] == 0) return;
   const int2 it = items[item];
   const int p = it.x;
   const int2 bk = buckets[it.y];
@@ -87,7 +89,7 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
   const double Qp = Q[p];
   const long long base64 = WRITE ? offsets[item] : 0;
   const long long base32 = WRITE ? offsets[nitems + item] : 0;
-  double* const js = zero_fill ? jbuf + jslot[item] : nullptr;
+  unsigned long long* const mw = masks ? jmask + mbase[item] : nullptr;
   int c64 = 0, c32 = 0, npre = 0;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int q0 = bk.x; q0 < end; q0 += 64) {
@@ -113,8 +115,7 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
       else
         tasks[base64 + c64 + __popcll(m64 & below)] = make_int2(p, q);
     }
-    if (zero_fill && pre && !ok)
-      for (int f = 0; f < nk; ++f) js[(size_t)(q - bk.x) * nk + f] = 0.0;
+    if (masks && lane == 0) mw[(q0 - bk.x) >> 6] = m64 | m32;
     c64 += __popcll(m64);
     c32 += __popcll(m32);
     npre += __popcll(__ballot(pre));
@@ -123,6 +124,7 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
     counts[item] = c64;
     counts[nitems + item] = c32;
     nslot[item] = npre * nk;
+    nmask[item] = (npre + 63) >> 6;
   }
 }
 
@@ -141,8 +143,8 @@ __global__ __launch_bounds__(256) void k_task_scan_seg(const int2* __restrict__ 
                                                        const long long* __restrict__ offsets,
                                                        int2* __restrict__ tasks, const int* __restrict__ item_cp,
                                                        const int* __restrict__ cp_nk, int* __restrict__ nslot,
-                                                       const long long* __restrict__ jslot,
-                                                       double* __restrict__ jbuf) {
+                                                       int* __restrict__ nmask, const long long* __restrict__ mbase,
+                                                       unsigned long long* __restrict__ jmask) {
   extern __shared__ float prow[];  // [2][nsh]: rows a and b of Pn
   const int2 sg = segs[blockIdx.x];
   const int2 ab = psh[items[sg.x].x];
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void k_task_scan_seg(const int2* __restrict__ 
   auto pn = [&](int c, int d) { return fmaxf(fmaxf(prow[c], prow[d]), fmaxf(prow[nsh + c], prow[nsh + d])); };
   for (int item = sg.x + (int)(threadIdx.x >> 6); item < sg.y; item += (int)(blockDim.x >> 6))
     scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, pab, pn, cut, thr, mix_thr,
-                     counts, offsets, tasks, nk, nslot, jslot, jbuf);
+                     counts, offsets, tasks, nk, nslot, nmask, mbase, jmask);
 }
 
 // The same scan with one wave per item and the norms gathered from HBM (for
@@ -174,7 +176,8 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
                                                    const long long* __restrict__ offsets,
                                                    int2* __restrict__ tasks, const int* __restrict__ item_cp,
                                                    const int* __restrict__ cp_nk, int* __restrict__ nslot,
-                                                   const long long* __restrict__ jslot, double* __restrict__ jbuf) {
+                                                   int* __restrict__ nmask, const long long* __restrict__ mbase,
+                                                   unsigned long long* __restrict__ jmask) {
   const long long item = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (item >= nitems) return;
@@ -183,8 +186,8 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
   const float* rb = Pn + (size_t)ab.y * nsh;
   auto pn = [&](int c, int d) { return fmaxf(fmaxf(ra[c], ra[d]), fmaxf(rb[c], rb[d])); };
   scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, ra[ab.y], pn,
-                   thr / (*pmax_ptr), thr, mix_thr, counts, offsets, tasks, cp_nk[item_cp[item]], nslot, jslot,
-                   jbuf);
+                   thr / (*pmax_ptr), thr, mix_thr, counts, offsets, tasks, cp_nk[item_cp[item]], nslot, nmask, mbase,
+                   jmask);
 }
 
 // Pcd[q] = Pn of the shell pair of pair q
@@ -269,7 +272,7 @@ void launch_pair_norm(const int2* psh, int npair, const float* Pn, int nsh, floa
 
 template <bool WRITE>
 static void task_scan(const ScanArgs& a, int* counts, const long long* offsets, int2* tasks, int* nslot,
-                      const long long* jslot, double* jbuf, hipStream_t s) {
+                      int* nmask, const long long* mbase, unsigned long long* jmask, hipStream_t s) {
   if (a.nitems <= 0) return;
   const size_t lds = 2 * sizeof(float) * (size_t)a.nsh;
   if (a.segs && a.nseg > 0 && lds <= (size_t)a.lds_max) {
@@ -277,22 +280,22 @@ static void task_scan(const ScanArgs& a, int* counts, const long long* offsets, 
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, dim3((unsigned)a.nseg), dim3(256), lds, s, a.segs, a.items, a.nitems, a.buckets,
                        a.pair_bucket, a.psh, a.Q, a.Pn, a.Pcd, a.nsh, a.pmax, a.thr, a.mix_thr, counts, offsets,
-                       tasks, a.item_cp, a.cp_nk, nslot, jslot, jbuf);
+                       tasks, a.item_cp, a.cp_nk, nslot, nmask, mbase, jmask);
   } else {
     const long long threads = a.nitems * 64;
     hipLaunchKernelGGL((k_task_scan<WRITE>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, a.items,
                        a.nitems, a.buckets, a.pair_bucket, a.psh, a.Q, a.Pn, a.Pcd, a.nsh, a.pmax, a.thr, a.mix_thr,
-                       counts, offsets, tasks, a.item_cp, a.cp_nk, nslot, jslot, jbuf);
+                       counts, offsets, tasks, a.item_cp, a.cp_nk, nslot, nmask, mbase, jmask);
   }
 }
 
-void launch_task_count(const ScanArgs& a, int* counts, int* nslot, hipStream_t s) {
-  task_scan<false>(a, counts, nullptr, nullptr, nslot, nullptr, nullptr, s);
+void launch_task_count(const ScanArgs& a, int* counts, int* nslot, int* nmask, hipStream_t s) {
+  task_scan<false>(a, counts, nullptr, nullptr, nslot, nmask, nullptr, nullptr, s);
 }
 
 void launch_task_write(const ScanArgs& a, const int* counts, const long long* offsets, int2* tasks,
-                       const long long* jslot, double* jbuf, hipStream_t s) {
-  task_scan<true>(a, (int*)counts, offsets, tasks, nullptr, jslot, jbuf, s);
+                       const long long* mbase, unsigned long long* jmask, hipStream_t s) {
+  task_scan<true>(a, (int*)counts, offsets, tasks, nullptr, nullptr, mbase, jmask, s);
 }
 
 // The scans and sums read the int counts widened to 64 bits: hipcub
@@ -345,12 +348,17 @@ void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
 // item0 + i * stride, i < np) against positions [j0, j0 + kJcdTileKet) of one
 // ket bucket whose first pair is q0.  One wave per tile: each lane owns one
 // position of a 64-wide column block, walks the tile's items (their prefix
-// lengths and slot bases are loaded once, four per lane, and broadcast), reads
-// its share with coalesced loads, and adds the column sums to J.
+// lengths, slot bases and mask bases are loaded once, four per lane), and for
+// each 64-position block gathers every item's survivor word (one per lane),
+// visits only the items with a survivor there (two at a time, their bases
+// broadcast with readlane), reads the surviving shares with coalesced loads,
+// and adds the column sums to J.
 template <int NK>
 __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tiles, int ntiles,
                                                    const int* __restrict__ nslot,
                                                    const long long* __restrict__ jslot,
+                                                   const long long* __restrict__ mbase,
+                                                   const unsigned long long* __restrict__ jmask,
                                                    const double* __restrict__ jbuf,
                                                    const PairHdr* __restrict__ pairs, int nsb,
                                                    double* __restrict__ J, int nbf) {
@@ -360,17 +368,19 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
   const int4 tl = tiles[w];
   const int stride = tl.y >> 16, np = tl.y & 0xffff;
   int nl[4];
-  long long bl[4];
+  long long bl[4], ml[4];
   int nmax = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int i = lane + 64 * k;
     nl[k] = 0;
     bl[k] = 0;
+    ml[k] = 0;
     if (i < np) {
       const int item = tl.x + i * stride;
       nl[k] = nslot[item] / NK;
       bl[k] = jslot[item];
+      ml[k] = mbase[item];
     }
     nmax = nl[k] > nmax ? nl[k] : nmax;
   }
@@ -387,18 +397,35 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
     for (int f = 0; f < NK; ++f) acc[f] = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int lim = np - 64 * k < 64 ? np - 64 * k : 64;
-      for (int l = 0; l < lim; ++l) {
-        const int n = __builtin_amdgcn_readlane(nl[k], l);
-        if (n <= jc) continue;  // the same for the whole wave
-        const int lo = __builtin_amdgcn_readlane((int)(bl[k] & 0xffffffffLL), l);
-        const int hi = __builtin_amdgcn_readlane((int)(bl[k] >> 32), l);
-        const long long b = ((long long)hi << 32) | (unsigned)lo;
-        if (j < n) {
-          const double* sh = jbuf + b + (long long)j * NK;
+      // each lane gathers its own item's survivor word for these 64 positions;
+      // the wave then visits only the items that have a survivor here
+      unsigned long long wk = 0ull;
+      if (nl[k] > jc) wk = jmask[ml[k] + (jc >> 6)];
+      unsigned long long act = __ballot(wk != 0ull);
+      while (act) {
+        // two items per step, so that their share loads overlap
+        const int l0 = __builtin_ctzll(act);
+        act &= act - 1ull;
+        const int l1 = act ? __builtin_ctzll(act) : l0;
+        if (act) act &= act - 1ull;
+        const unsigned long long w0 = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wk >> 32), l0) << 32) |
+                                      (unsigned)__builtin_amdgcn_readlane((int)(wk & 0xffffffffull), l0);
+        const unsigned long long w1 = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wk >> 32), l1) << 32) |
+                                      (unsigned)__builtin_amdgcn_readlane((int)(wk & 0xffffffffull), l1);
+        const long long b0 = ((long long)__builtin_amdgcn_readlane((int)(bl[k] >> 32), l0) << 32) |
+                             (unsigned)__builtin_amdgcn_readlane((int)(bl[k] & 0xffffffffLL), l0);
+        const long long b1 = ((long long)__builtin_amdgcn_readlane((int)(bl[k] >> 32), l1) << 32) |
+                             (unsigned)__builtin_amdgcn_readlane((int)(bl[k] & 0xffffffffLL), l1);
+        const bool t0 = (w0 >> lane) & 1ull;
+        const bool t1 = l1 != l0 && ((w1 >> lane) & 1ull);
+        double v0[NK], v1[NK];
 #pragma unroll
-          for (int f = 0; f < NK; ++f) acc[f] += sh[f];
+        for (int f = 0; f < NK; ++f) {
+          v0[f] = t0 ? jbuf[b0 + (long long)j * NK + f] : 0.0;
+          v1[f] = t1 ? jbuf[b1 + (long long)j * NK + f] : 0.0;
         }
+#pragma unroll
+        for (int f = 0; f < NK; ++f) acc[f] += v0[f] + v1[f];
       }
     }
     if (j < nmax) {
@@ -412,26 +439,28 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
 
 template <int NK>
 static void jcd_tiles_launch(const int4* tiles, int ntiles, const int* nslot, const long long* jslot,
-                             const double* jbuf, const PairHdr* pairs, int nsb, double* J, int nbf, hipStream_t s) {
+                             const long long* mbase, const unsigned long long* jmask, const double* jbuf,
+                             const PairHdr* pairs, int nsb, double* J, int nbf, hipStream_t s) {
   const long long threads = (long long)ntiles * 64;
   hipLaunchKernelGGL((k_jcd_tiles<NK>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tiles, ntiles,
-                     nslot, jslot, jbuf, pairs, nsb, J, nbf);
+                     nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf);
 }
 
 void launch_jcd_tiles(const int4* tiles, int ntiles, int nk, int nsb, const int* nslot, const long long* jslot,
-                      const double* jbuf, const PairHdr* pairs, double* J, int nbf, hipStream_t s) {
+                      const long long* mbase, const unsigned long long* jmask, const double* jbuf,
+                      const PairHdr* pairs, double* J, int nbf, hipStream_t s) {
   if (ntiles <= 0) return;
   switch (nk) {
-    case 1: jcd_tiles_launch<1>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 3: jcd_tiles_launch<3>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 5: jcd_tiles_launch<5>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 7: jcd_tiles_launch<7>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 9: jcd_tiles_launch<9>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 15: jcd_tiles_launch<15>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 21: jcd_tiles_launch<21>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 25: jcd_tiles_launch<25>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 35: jcd_tiles_launch<35>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
-    case 49: jcd_tiles_launch<49>(tiles, ntiles, nslot, jslot, jbuf, pairs, nsb, J, nbf, s); break;
+    case 1: jcd_tiles_launch<1>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 3: jcd_tiles_launch<3>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 5: jcd_tiles_launch<5>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 7: jcd_tiles_launch<7>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 9: jcd_tiles_launch<9>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 15: jcd_tiles_launch<15>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 21: jcd_tiles_launch<21>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 25: jcd_tiles_launch<25>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 35: jcd_tiles_launch<35>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 49: jcd_tiles_launch<49>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
     default: break;  // no other pure pair sizes up to f
   }
 }

@@ -37,12 +37,13 @@ struct ScanArgs {
   int lds_max;
 };
 void launch_pair_norm(const int2* psh, int npair, const float* Pn, int nsh, float* Pcd, hipStream_t s);
-// counts [2][nitems] (FP64, FP32 survivors) and nslot[nitems] (J_cd slots)
-void launch_task_count(const ScanArgs& a, int* counts, int* nslot, hipStream_t s);
-// tasks at the scanned offsets; with jbuf, zero J_cd shares for the prefix
-// quartets that fail the density test (slot bases jslot)
+// counts [2][nitems] (FP64, FP32 survivors), nslot[nitems] (J_cd slots) and
+// nmask[nitems] (64-bit survivor-mask words)
+void launch_task_count(const ScanArgs& a, int* counts, int* nslot, int* nmask, hipStream_t s);
+// tasks at the scanned offsets; with jmask, each item's survivor mask from
+// its word base mbase[item]
 void launch_task_write(const ScanArgs& a, const int* counts, const long long* offsets, int2* tasks,
-                       const long long* jslot, double* jbuf, hipStream_t s);
+                       const long long* mbase, unsigned long long* jmask, hipStream_t s);
 // workspace of launch_exclusive_scan and launch_sum for up to n ints (both
 // accumulate in 64 bits)
 size_t scan_temp_bytes(long long n);
@@ -59,14 +60,15 @@ void launch_symmetrize(double* J, double* K, int n, hipStream_t s);
 
 // J_cd by ket pair.  Item i (bra pair p, ket bucket kb) owns nslot[i] / nk
 // share slots of nk doubles from jbuf + jslot[i], one per position j of the
-// Schwarz prefix of its bucket range (ket pair = first pair of kb + j).  The
-// class kernels store the shares of the surviving quartets, the task-write
-// pass zeros the others.  launch_jcd_tiles sums them per ket pair into J for
-// the tiles of one ket pair size nk (nsb = functions of the pair's second
-// shell); a tile is int4 {first item, stride << 16 | bra pairs (<= 256),
-// first pair of the ket bucket, first position}.
+// Schwarz prefix of its bucket range (ket pair = first pair of kb + j), and
+// survivor-mask words from jmask + mbase[i].  The class kernels store the
+// shares of the surviving quartets.  launch_jcd_tiles sums them per ket pair
+// into J for the tiles of one ket pair size nk (nsb = functions of the pair's
+// second shell); a tile is int4 {first item, stride << 16 | bra pairs
+// (<= 256), first pair of the ket bucket, first position (a multiple of 64)}.
 void launch_jcd_tiles(const int4* tiles, int ntiles, int nk, int nsb, const int* nslot, const long long* jslot,
-                      const double* jbuf, const PairHdr* pairs, double* J, int nbf, hipStream_t s);
+                      const long long* mbase, const unsigned long long* jmask, const double* jbuf,
+                      const PairHdr* pairs, double* J, int nbf, hipStream_t s);
 constexpr int kJcdTileBra = 256;   // bra pairs per tile
 constexpr int kJcdTileKet = 1024;  // ket positions per tile
 

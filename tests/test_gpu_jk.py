@@ -119,9 +119,10 @@ def test_jcd_slots_and_atomic_paths_agree(monkeypatch, mix):
 
 def test_jcd_slots_zeroed_between_builds(monkeypatch):
     """Share slots of quartets that pass the Schwarz bound but fail the
-    density test are zeroed each build: after a dense density, a
-    block-diagonal one (no density between molecules, so many such quartets)
-    gives the same J as the atomic path and the oracle."""
+    density test keep the previous build's values; the survivor masks leave
+    them out: after a dense density, a block-diagonal one (no density between
+    molecules, so many such quartets) gives the same J as the atomic path and
+    the oracle."""
     from oracle.oracle import OracleERI
     from qdk_chemistry_amd.basis import load_basis
     from qdk_chemistry_amd.eri import create
