@@ -245,6 +245,7 @@ def main():
         mixed["speedup_vs_fp64"] = s64["ms_per_build"] / s32["ms_per_build"]
         log(f"mixed timed: {s32['ms_per_build']:.2f} ms/build ({mixed['speedup_vs_fp64']:.2f}x), "
             f"FP32 share {s32['fp32_fraction']:.3f}, max|dF| {mixed['max_abs_dF_vs_fp64']:.2e}")
+        log_classes(run32, args.steps)
         if "scf_fp64" in result:  # the mixed SCF is compared with the FP64 one run above
             t = time.perf_counter()
             eng.set_precision(True, 1e-6)
