@@ -84,7 +84,9 @@ class OracleERI:
         self.structure = structure
         l, atom, cen, po, exps, coefs = basis.flat()
         self._keep = (l, atom, cen, po, exps, coefs)
-        Z = np.ascontiguousarray(structure.numbers.astype(np.float64))
+        # nuclear charges seen by the electrons (Z minus ECP core electrons)
+        Z = np.ascontiguousarray(np.asarray(structure.numbers, dtype=np.float64)
+                                 - np.asarray(getattr(basis, "ecp_electrons", 0), dtype=np.float64))
         R = np.ascontiguousarray(structure.coords.astype(np.float64))
         self._keepZR = (Z, R)
         h = lib().oracle_create(basis.nsh, _p(l, ctypes.c_int), _p(atom, ctypes.c_int), _p(cen),

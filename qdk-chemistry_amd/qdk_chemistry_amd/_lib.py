@@ -97,7 +97,7 @@ class BasisArrays:
     def __init__(self, basis, structure):
         l, atom, cen, po, exps, coefs = basis.flat()
         self.arrays = dict(l=l, atom=atom, cen=cen, po=po, exps=exps, coefs=coefs,
-                           Z=np.ascontiguousarray(structure.numbers.astype(np.float64)),
+                           Z=np.ascontiguousarray(basis.effective_charges(structure.numbers)),
                            R=np.ascontiguousarray(structure.coords.astype(np.float64)))
         a = self.arrays
         self.c = QfeBasis(basis.nsh, ptr(a["l"], ctypes.c_int), ptr(a["atom"], ctypes.c_int), ptr(a["cen"]),

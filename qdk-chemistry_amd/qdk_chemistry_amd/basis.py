@@ -88,12 +88,22 @@ class BasisSet:
     pure: bool = True
     natoms: int = 0
     shell_offsets: np.ndarray = field(default=None)
+    # core electrons an effective core potential replaces, per atom (the
+    # nuclear charges of the one- and two-electron Hamiltonian are reduced
+    # by them: core/basis_set.cpp:85-88)
+    ecp_electrons: np.ndarray = field(default=None)
 
     def __post_init__(self):
         offs = [0]
         for sh in self.shells:
             offs.append(offs[-1] + self.nfun(sh.l))
         self.shell_offsets = np.array(offs, dtype=np.int64)
+        if self.ecp_electrons is None:
+            self.ecp_electrons = np.zeros(self.natoms, dtype=np.int64)
+
+    def effective_charges(self, numbers) -> np.ndarray:
+        """Nuclear charges seen by the electrons: Z minus the ECP core electrons."""
+        return np.asarray(numbers, dtype=np.float64) - self.ecp_electrons
 
     def nfun(self, l: int) -> int:
         return 2 * l + 1 if (self.pure and l >= 2) else (l + 1) * (l + 2) // 2
@@ -133,10 +143,12 @@ def load_basis(name: str, numbers, coords_bohr, pure: bool = True, sort: bool = 
         data = json.load(f)
     coords = np.asarray(coords_bohr, dtype=np.float64).reshape(-1, 3)
     shells = []
+    ecp = np.zeros(len(numbers), dtype=np.int64)
     for ia, z in enumerate(numbers):
         key = str(int(z))
         if key not in data["elements"]:
             raise ValueError(f"Element (Z={key}) is not supported in basis set {name}")
+        ecp[ia] = int(data["elements"][key].get("ecp_electrons", 0))
         for entry in data["elements"][key]["electron_shells"]:
             ams = entry["angular_momentum"]
             exps = np.array([float(x) for x in entry["exponents"]])
@@ -150,7 +162,7 @@ def load_basis(name: str, numbers, coords_bohr, pure: bool = True, sort: bool = 
         shells = sorted(shells, key=lambda s: (s.l, -s.exps[0]))
     for s in shells:
         s.coefs = normalize_psi4(s.l, s.exps, s.coefs)
-    return BasisSet(name=name.lower(), shells=shells, pure=pure, natoms=len(numbers))
+    return BasisSet(name=name.lower(), shells=shells, pure=pure, natoms=len(numbers), ecp_electrons=ecp)
 
 
 # ---------------------------------------------------------------------------

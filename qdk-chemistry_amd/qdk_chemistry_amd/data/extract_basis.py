@@ -1,9 +1,12 @@
-"""Extract H/Li/C/N/O subsets of the basis-set-exchange JSON files that
+"""Extract H/Li/C/N/O subsets (and Te/I of def2-SVP, with their ECP core
+electron counts) of the basis-set-exchange JSON files that
 qdk-chemistry ships (cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/
 compressed/<name>.tar.gz, member basis/<name>.json) into data/basis/.
 
-Only `electron_shells` of the kept elements are copied; the values are data
-(exponents and contraction coefficients as strings, untouched).  Run once in a
+Only `electron_shells` (and `ecp_electrons`) of the kept elements are copied;
+the values are data (exponents and contraction coefficients as strings,
+untouched).  The ECP potentials themselves are not copied: ECP integrals are
+not part of this package, only the reduced nuclear charges they imply.  Run once in a
 container that holds the reference checkout; the output JSON is committed.
 """
 import io
@@ -15,6 +18,7 @@ import tarfile
 REF = "/root/reference/cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/compressed"
 NAMES = ["cc-pvdz", "cc-pvtz", "def2-svp", "def2-tzvp", "6-31g_st_", "sto-3g"]
 ELEMENTS = ["1", "3", "6", "7", "8"]
+EXTRA = {"def2-svp": ["52", "53"]}  # Te, I: the one-electron integral fixture's molecule
 
 
 def main(out_dir):
@@ -24,9 +28,12 @@ def main(out_dir):
             member = tf.getmember(f"basis/{name}.json")
             data = json.load(io.TextIOWrapper(tf.extractfile(member)))
         keep = {"name": data.get("name", name), "elements": {}}
-        for z in ELEMENTS:
+        for z in ELEMENTS + EXTRA.get(name, []):
             if z in data["elements"]:
-                keep["elements"][z] = {"electron_shells": data["elements"][z]["electron_shells"]}
+                el = data["elements"][z]
+                keep["elements"][z] = {"electron_shells": el["electron_shells"]}
+                if "ecp_electrons" in el:
+                    keep["elements"][z]["ecp_electrons"] = el["ecp_electrons"]
         with open(os.path.join(out_dir, name + ".json"), "w") as f:
             json.dump(keep, f, indent=0)
 

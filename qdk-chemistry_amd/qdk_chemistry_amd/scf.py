@@ -112,6 +112,8 @@ def orthogonalizer(S, lindep=-1.0):
 
 class RHF:
     def __init__(self, structure, basis, cfg: SCFConfig | None = None, eri=None):
+        if np.any(getattr(basis, "ecp_electrons", 0)):
+            raise NotImplementedError("effective core potentials: the ECP integrals are not part of this package")
         self.structure = structure
         self.basis = basis
         self.cfg = cfg or SCFConfig()

@@ -1,9 +1,13 @@
-"""Copy the reference's RHF gradient fixtures into reference_fixtures.json.
+"""Copy the reference's RHF gradient fixtures into reference_fixtures.json,
+and its one-electron integral fixture into int1e_teh2hi.npz.
 
 Sources (data only): scf/tests/data/h2o.json and water10.json (RHF/HF/6-31g*/
 pure energy + gradient, atom-major), geometries from scf/tests/test_common.h
-(make_h2o, make_water10; angstrom with the local bohr factor 0.52917721092).
-Run in a checkout that holds the reference; the JSON output is committed.
+(make_h2o, make_water10, make_TeH2_HI; angstrom with the local bohr factor
+0.52917721092); scf/tests/data/int1e.json (Psi4 S, T, V of TeH2-HI in
+def2-SVP, cartesian, unsorted shells, with the density `dm` and the three
+gradients sum dm dO/dR, natoms x 3).  Run in a checkout that holds the
+reference; the outputs are committed.
 """
 import json
 import os
@@ -35,6 +39,17 @@ def main():
         out[name] = dict(numbers=Z, coords_angstrom=R, basis="6-31g*", pure=True,
                          energy=float(rec["energy"]), gradient=[float(g) for g in grad])
     json.dump(out, open(os.path.join(HERE, "reference_fixtures.json"), "w"), indent=1)
+    import numpy as np
+    rec = json.load(open(os.path.join(ROOT, "data", "int1e.json")))
+    assert rec["mol"] == "TeH2-HI" and rec["basis"] == "def2-svp"
+    Z, R = coords_from_header("make_TeH2_HI")
+    n = int(round(len(rec["overlap_integral"]) ** 0.5))
+    arrs = dict(numbers=np.array(Z), coords_angstrom=np.array(R))
+    for k in ("overlap_integral", "kinetic_integral", "nuclear_integral", "dm"):
+        arrs[k] = np.array(rec[k], dtype=np.float64).reshape(n, n)
+    for k in ("overlap_gradient", "kinetic_gradient", "nuclear_gradient"):
+        arrs[k] = np.array(rec[k], dtype=np.float64).reshape(len(Z), 3)
+    np.savez_compressed(os.path.join(HERE, "int1e_teh2hi.npz"), **arrs)
 
 
 if __name__ == "__main__":

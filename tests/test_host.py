@@ -166,3 +166,36 @@ def test_eri_multiplexer_routes_like_the_reference():
     calls.clear()
     J, K = ERIMultiplexer(Fake("j"), Fake("k")).build_JK(P, 1.0, 0.0, 0.3)
     assert calls == [("j", 0.0), ("k", 0.3)] and J[0, 0] == 1.0 and K[0, 0] == 4.0
+
+
+_INT1E = [(0, "overlap", 1e-9), (1, "kinetic", 1e-9), (2, "nuclear", 1e-8)]
+
+
+def _teh2hi():
+    """TeH2-HI in def2-SVP, cartesian, unsorted shells: the molecule and basis
+    of the reference's one-electron integral test (int1e_test.cpp:38-46)."""
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.molecule import Structure
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden", "int1e_teh2hi.npz"))
+    mol = Structure(f["numbers"], f["coords_angstrom"] / 0.52917721092)
+    return f, mol, load_basis("def2-svp", mol.numbers, mol.coords, pure=False, sort=False)
+
+
+@pytest.mark.parametrize("op,name,tol", _INT1E)
+def test_one_body_vs_reference_int1e_fixture(op, name, tol):
+    """S, T and V (Te and I with their 28-electron ECP cores: V sees Z - 28,
+    core/basis_set.cpp:85-87) against the reference's Psi4 values, with its
+    tolerances (int1e_test.cpp:96-99)."""
+    from qdk_chemistry_amd.eri import one_body
+    f, mol, b = _teh2hi()
+    assert b.nbf == f["dm"].shape[0] and list(b.ecp_electrons) == [28, 0, 0, 0, 28]
+    assert np.max(np.abs(one_body(b, mol, op) - f[name + "_integral"])) < tol
+
+
+@pytest.mark.parametrize("op,name,tol", _INT1E)
+def test_one_body_gradient_vs_reference_int1e_fixture(op, name, tol):
+    """sum_mn D_mn dO_mn/dR for S, T and V against the reference's values
+    (int1e_test.cpp:101-105: the same tolerances)."""
+    from qdk_chemistry_amd.eri import one_body_grad
+    f, mol, b = _teh2hi()
+    assert np.max(np.abs(one_body_grad(b, mol, op, f["dm"]) - f[name + "_gradient"])) < tol
