@@ -114,6 +114,16 @@ int qfe_one_body(const qfe_basis* basis, int op, double* out);
 /* Gradient of sum_mn W_mn d(O)_mn / dR for op 0/1/2 (3*natoms, atom-major). */
 int qfe_one_body_grad(const qfe_basis* basis, int op, const double* W, double* grad);
 
+/* The same two on GPU device_id (host inputs and outputs; shells up to f).
+ * Replace OneBodyIntegral::overlap_integral / kinetic_integral /
+ * nuclear_integral (scf/include/qdk/chemistry/scf/util/int1e.h; Libint2
+ * evaluation in scf/src/util/int1e.cpp:473-977) and their *_deriv(dm, res)
+ * contractions (the same file).  The nuclear charges are those of the basis
+ * struct: Z minus the ECP core electrons, as the reference's atomic_charges
+ * (core/basis_set.cpp:85-87). */
+int qfe_one_body_gpu(const qfe_basis* basis, int op, double* out, int device_id);
+int qfe_one_body_grad_gpu(const qfe_basis* basis, int op, const double* W, double* grad, int device_id);
+
 /* Switch the precision policy of subsequent builds: mix != 0 evaluates
  * quartets with Q_ab Q_cd < threshold in FP32 (ERIConfig::is_mix_precision,
  * core/scf.h:71-93, which the reference declares but no shipped engine reads);

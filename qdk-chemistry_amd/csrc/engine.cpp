@@ -19,6 +19,7 @@
 #include <dlfcn.h>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -29,6 +30,7 @@
 #include "eri_core.hpp"
 #include "host_common.hpp"
 #include "kernels.hpp"
+#include "onebody_gpu.hpp"
 #include "screen.hpp"
 
 namespace qfe {
@@ -906,6 +908,84 @@ struct Engine {
   }
 };
 
+// ------------------------------------------------------------ one-electron integrals on the GPU
+// Device copies of the Rys and Boys tables and of the solid-harmonic table
+// for the one-electron kernels: built once per device and kept until the
+// process ends (about 0.5 MB; freeing them at exit would race the runtime's
+// own teardown).
+struct OneBodyTables {
+  RysArgs ra;
+  const double* sph;
+};
+static double* upload_forever(const std::vector<double>& v) {
+  double* p = nullptr;
+  HIP_CHECK(hipMalloc((void**)&p, std::max<size_t>(v.size(), 1) * sizeof(double)));
+  if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  return p;
+}
+static const OneBodyTables& one_body_tables(int device) {
+  static std::mutex mu;
+  static std::map<int, OneBodyTables> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(device);
+  if (it != cache.end()) return it->second;
+  OneBodyTables t;
+  std::memset(&t.ra, 0, sizeof(t.ra));
+  const RysTableHost& rt = rys_table();
+  for (int n = 1; n <= rt.nmax && n <= 8; ++n) {
+    t.ra.coef[n] = upload_forever(rt.coef[n]);
+    t.ra.tmax[n] = rt.tmax[n];
+    for (int i = 0; i < n; ++i) {
+      t.ra.hx2[n][i] = rt.hx2[n][i];
+      t.ra.hw[n][i] = rt.hw[n][i];
+    }
+  }
+  t.ra.boys = upload_forever(Engine::boys_table());
+  const double* k0 = &::kSph[0][0][0];
+  t.sph = upload_forever(std::vector<double>(k0, k0 + sizeof(::kSph) / sizeof(double)));
+  return cache.emplace(device, t).first->second;
+}
+
+// flat copy of a basis on the current device, for one call
+struct OneBodyDevice {
+  DevBuf<int> l, atom, po, bf0;
+  DevBuf<double> cen, exps, coefs, Z, R;
+  OneBodyView view{};
+  OneBodyDevice(const BasisHost& b, const double* sph) {
+    std::vector<int> hl(b.nsh), hat(b.nsh), hpo(b.nsh + 1, 0), hbf(b.nsh);
+    std::vector<double> hc(3 * (size_t)b.nsh), he, hk;
+    for (int s = 0; s < b.nsh; ++s) {
+      const ShellHost& sh = b.shells[s];
+      if (sh.l > 3) throw Unsupported("one-electron GPU integrals are compiled for shells up to f");
+      hl[s] = sh.l;
+      hat[s] = sh.atom;
+      hbf[s] = b.bf0[s];
+      for (int d = 0; d < 3; ++d) hc[3 * s + d] = sh.O[d];
+      he.insert(he.end(), sh.a.begin(), sh.a.end());
+      hk.insert(hk.end(), sh.c.begin(), sh.c.end());
+      hpo[s + 1] = (int)he.size();
+    }
+    l.upload(hl, nullptr);
+    atom.upload(hat, nullptr);
+    po.upload(hpo, nullptr);
+    bf0.upload(hbf, nullptr);
+    cen.upload(hc, nullptr);
+    exps.upload(he, nullptr);
+    coefs.upload(hk, nullptr);
+    Z.upload(b.Z, nullptr);
+    R.upload(b.R, nullptr);
+    view = OneBodyView{l.p, atom.p, po.p, bf0.p, cen.p, exps.p, coefs.p, Z.p, R.p, sph, b.nsh, b.natoms, b.nbf,
+                       b.pure ? 1 : 0};
+  }
+};
+
+static void one_body_device(int device_id) {
+  int ndev = 0;
+  HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (device_id < 0 || device_id >= ndev) throw std::invalid_argument("device_id out of range");
+  HIP_CHECK(hipSetDevice(device_id));
+}
+
 }  // namespace qfe
 
 // ------------------------------------------------------------ C ABI
@@ -1053,6 +1133,42 @@ int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, doubl
     if (ms) ms[cp] = e.cls_ms.empty() ? 0.0 : e.cls_ms[(precision ? ncp : 0) + cp];
   }
   return ncp;
+}
+
+int qfe_one_body_gpu(const qfe_basis* basis, int op, double* out, int device_id) {
+  return guarded([&] {
+    if (op < 0 || op > 2) throw std::invalid_argument("op must be 0 (S), 1 (T) or 2 (V)");
+    if (!out) throw std::invalid_argument("null output");
+    const qfe::BasisHost b = qfe::BasisHost::from_c(basis);
+    qfe::one_body_device(device_id);
+    const qfe::OneBodyTables& tb = qfe::one_body_tables(device_id);
+    qfe::OneBodyDevice d(b, tb.sph);
+    qfe::DevBuf<double> o;
+    o.alloc((size_t)b.nbf * b.nbf);
+    qfe::launch_one_body(d.view, tb.ra, op, o.p, nullptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(out, o.p, sizeof(double) * (size_t)b.nbf * b.nbf, hipMemcpyDeviceToHost));
+  });
+}
+
+int qfe_one_body_grad_gpu(const qfe_basis* basis, int op, const double* W, double* grad, int device_id) {
+  return guarded([&] {
+    if (op < 0 || op > 2) throw std::invalid_argument("op must be 0 (S), 1 (T) or 2 (V)");
+    if (!W || !grad) throw std::invalid_argument("null input or output");
+    const qfe::BasisHost b = qfe::BasisHost::from_c(basis);
+    qfe::one_body_device(device_id);
+    const qfe::OneBodyTables& tb = qfe::one_body_tables(device_id);
+    qfe::OneBodyDevice d(b, tb.sph);
+    const size_t nn = (size_t)b.nbf * b.nbf;
+    qfe::DevBuf<double> w, g;
+    w.alloc(nn);
+    g.alloc(3 * (size_t)b.natoms);
+    HIP_CHECK(hipMemcpy(w.p, W, sizeof(double) * nn, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(g.p, 0, sizeof(double) * 3 * (size_t)b.natoms));
+    qfe::launch_one_body_grad(d.view, tb.ra, op, w.p, g.p, nullptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(grad, g.p, sizeof(double) * 3 * (size_t)b.natoms, hipMemcpyDeviceToHost));
+  });
 }
 
 int qfe_max_l(void) { return QFE_MAX_L; }

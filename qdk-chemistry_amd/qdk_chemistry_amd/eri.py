@@ -189,3 +189,22 @@ def one_body_grad(basis, structure, op: int, W) -> np.ndarray:
     g = np.zeros((structure.natoms, 3))
     _lib.check(_lib.lib().qfe_one_body_grad(ctypes.byref(b.c), op, _lib.ptr(W), _lib.ptr(g)))
     return g
+
+
+def one_body_gpu(basis, structure, op: int, device_id: int = 0) -> np.ndarray:
+    """S (0), T (1) or V (2) computed on the GPU (OneBodyIntegral::
+    overlap_integral / kinetic_integral / nuclear_integral, int1e.h)."""
+    b = _lib.BasisArrays(basis, structure)
+    out = np.zeros((basis.nbf, basis.nbf))
+    _lib.check(_lib.lib().qfe_one_body_gpu(ctypes.byref(b.c), op, _lib.ptr(out), device_id))
+    return out
+
+
+def one_body_grad_gpu(basis, structure, op: int, W, device_id: int = 0) -> np.ndarray:
+    """sum_mn W_mn dO_mn/dR on the GPU, natoms x 3 (the *_integral_deriv
+    contractions of int1e.h)."""
+    b = _lib.BasisArrays(basis, structure)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    g = np.zeros((structure.natoms, 3))
+    _lib.check(_lib.lib().qfe_one_body_grad_gpu(ctypes.byref(b.c), op, _lib.ptr(W), _lib.ptr(g), device_id))
+    return g

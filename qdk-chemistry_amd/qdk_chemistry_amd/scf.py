@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .eri import ERIConfig, create, one_body, one_body_grad
+from .eri import ERIConfig, create, one_body, one_body_gpu, one_body_grad, one_body_grad_gpu
 
 
 @dataclass
@@ -40,6 +40,20 @@ class SCFConfig:
     # diagonalisation are FP64 throughout)
     fp64_switch_gradient: float = 1e-5
     eri: ERIConfig = field(default_factory=ERIConfig)
+    # S, T, V and their derivatives on the ERI engine's GPU (else on the host)
+    one_body_on_gpu: bool = True
+
+
+def _one_body(basis, structure, op, cfg):
+    if cfg.one_body_on_gpu:
+        return one_body_gpu(basis, structure, op, cfg.eri.device_id)
+    return one_body(basis, structure, op)
+
+
+def _one_body_grad(basis, structure, op, W, cfg):
+    if cfg.one_body_on_gpu:
+        return one_body_grad_gpu(basis, structure, op, W, cfg.eri.device_id)
+    return one_body_grad(basis, structure, op, W)
 
 
 def _fractional_occupations(eps, nelec):
@@ -74,8 +88,9 @@ def atom_guess(structure, basis_name, pure=True, eri_cfg=None, iterations=30):
         at = Structure(np.array([z]), np.zeros((1, 3)))
         b = load_basis(basis_name, at.numbers, at.coords, pure=pure)
         eri = create(b, at, eri_cfg or ERIConfig())
-        S = one_body(b, at, 0)
-        H = one_body(b, at, 1) + one_body(b, at, 2)
+        dev = (eri_cfg or ERIConfig()).device_id
+        S = one_body_gpu(b, at, 0, dev)
+        H = one_body_gpu(b, at, 1, dev) + one_body_gpu(b, at, 2, dev)
         X = orthogonalizer(S)
         e, Cp = np.linalg.eigh(X.T @ H @ X)
         C = X @ Cp
@@ -121,8 +136,8 @@ class RHF:
         self.nocc = structure.nelectrons // 2
         if structure.nelectrons % 2:
             raise ValueError("RHF needs an even number of electrons")
-        self.S = one_body(basis, structure, 0)
-        self.H = one_body(basis, structure, 1) + one_body(basis, structure, 2)
+        self.S = _one_body(basis, structure, 0, self.cfg)
+        self.H = _one_body(basis, structure, 1, self.cfg) + _one_body(basis, structure, 2, self.cfg)
         self.X = orthogonalizer(self.S, self.cfg.lindep_threshold)
         self.E_nuc = structure.nuclear_repulsion()
         self.build_times = []
@@ -250,7 +265,8 @@ class RHF:
         W = 0.5 * P @ F @ P  # energy-weighted density (scf_impl.cpp:785-798 holds -W)
         b, m = self.basis, self.structure
         g = m.nuclear_repulsion_gradient()
-        g = g + one_body_grad(b, m, 1, P) + one_body_grad(b, m, 2, P) - one_body_grad(b, m, 0, W)
+        c = self.cfg
+        g = g + _one_body_grad(b, m, 1, P, c) + _one_body_grad(b, m, 2, P, c) - _one_body_grad(b, m, 0, W, c)
         dJ, dK = self.eri.get_gradients(P, 1.0, 0.0, 0.0)
         na = m.natoms
         self.grad_parts = dict(dJ=dJ.reshape(3, na).T.copy(), dK=dK.reshape(3, na).T.copy())
