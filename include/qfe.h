@@ -124,6 +124,13 @@ int qfe_one_body_grad(const qfe_basis* basis, int op, const double* W, double* g
 int qfe_one_body_gpu(const qfe_basis* basis, int op, double* out, int device_id);
 int qfe_one_body_grad_gpu(const qfe_basis* basis, int op, const double* W, double* grad, int device_id);
 
+/* Eigenvalues w (ascending) and eigenvectors V (row-major, eigenvector j in
+ * column j) of the symmetric n x n matrix A, on GPU device_id with
+ * rocSOLVER dsyevd.  Stands in for the SCF's diagonalisation of the
+ * orthogonalised Fock matrix (the reference's optional device path,
+ * scf_algorithm.cpp:105-152, scf_impl.cpp:431-449, and its LAPACK one). */
+int qfe_eigh_gpu(int n, const double* A, double* w, double* V, int device_id);
+
 /* Switch the precision policy of subsequent builds: mix != 0 evaluates
  * quartets with Q_ab Q_cd < threshold in FP32 (ERIConfig::is_mix_precision,
  * core/scf.h:71-93, which the reference declares but no shipped engine reads);

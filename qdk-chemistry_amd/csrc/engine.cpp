@@ -26,6 +26,9 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/qfe.h"
 #include "eri_core.hpp"
 #include "host_common.hpp"
@@ -979,6 +982,19 @@ struct OneBodyDevice {
   }
 };
 
+// one rocBLAS handle per device for the eigensolver, kept until the process
+// ends (creating one costs far more than a solve of SCF size)
+static rocblas_handle solver_handle(int device) {
+  static std::mutex mu;
+  static std::map<int, rocblas_handle> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(device);
+  if (it != cache.end()) return it->second;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) throw DeviceError("rocblas_create_handle failed");
+  return cache.emplace(device, h).first->second;
+}
+
 static void one_body_device(int device_id) {
   int ndev = 0;
   HIP_CHECK(hipGetDeviceCount(&ndev));
@@ -1168,6 +1184,36 @@ int qfe_one_body_grad_gpu(const qfe_basis* basis, int op, const double* W, doubl
     qfe::launch_one_body_grad(d.view, tb.ra, op, w.p, g.p, nullptr);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(grad, g.p, sizeof(double) * 3 * (size_t)b.natoms, hipMemcpyDeviceToHost));
+  });
+}
+
+int qfe_eigh_gpu(int n, const double* A, double* w, double* V, int device_id) {
+  return guarded([&] {
+    if (n <= 0 || !A || !w || !V) throw std::invalid_argument("eigh: need n > 0 and non-null arrays");
+    qfe::one_body_device(device_id);
+    rocblas_handle h = qfe::solver_handle(device_id);
+    const size_t nn = (size_t)n * n;
+    qfe::DevBuf<double> a, d, e;
+    qfe::DevBuf<rocblas_int> info;
+    a.alloc(nn);
+    d.alloc(n);
+    e.alloc(n);
+    info.alloc(1);
+    // A is symmetric: its row-major and column-major layouts are the same
+    HIP_CHECK(hipMemcpy(a.p, A, sizeof(double) * nn, hipMemcpyHostToDevice));
+    if (rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, a.p, n, d.p, e.p, info.p) !=
+        rocblas_status_success)
+      throw qfe::DeviceError("rocsolver_dsyevd failed");
+    rocblas_int hinfo = 0;
+    HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
+    if (hinfo != 0) throw std::runtime_error("eigh: the divide-and-conquer solve did not converge");
+    HIP_CHECK(hipMemcpy(w, d.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    // eigenvector j is column j of the column-major result: transpose into
+    // row-major V with the eigenvectors as columns
+    std::vector<double> cm(nn);
+    HIP_CHECK(hipMemcpy(cm.data(), a.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) V[(size_t)i * n + j] = cm[(size_t)j * n + i];
   });
 }
 

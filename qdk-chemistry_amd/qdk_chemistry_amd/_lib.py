@@ -39,7 +39,7 @@ class QfeStats(ctypes.Structure):
 
 
 EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients", "qfe_one_body",
-           "qfe_one_body_grad", "qfe_one_body_gpu", "qfe_one_body_grad_gpu", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_max_l", "qfe_destroy",
+           "qfe_one_body_grad", "qfe_one_body_gpu", "qfe_one_body_grad_gpu", "qfe_eigh_gpu", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_max_l", "qfe_destroy",
            "qfe_last_error"]
 
 _lib = None
@@ -70,6 +70,7 @@ def lib():
         L.qfe_one_body_grad.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, _dp]
         L.qfe_one_body_gpu.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, ctypes.c_int]
         L.qfe_one_body_grad_gpu.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.qfe_eigh_gpu.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
         L.qfe_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(QfeStats)]
         L.qfe_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.qfe_device_buffers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),

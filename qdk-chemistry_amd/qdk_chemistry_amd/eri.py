@@ -191,6 +191,19 @@ def one_body_grad(basis, structure, op: int, W) -> np.ndarray:
     return g
 
 
+def eigh_gpu(A, device_id: int = 0):
+    """(w, V) of a symmetric matrix on the GPU (rocSOLVER dsyevd): w ascending,
+    eigenvector j in column j, as numpy.linalg.eigh."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    if A.shape != (n, n):
+        raise ValueError("eigh_gpu needs a square matrix")
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    _lib.check(_lib.lib().qfe_eigh_gpu(n, _lib.ptr(A), _lib.ptr(w), _lib.ptr(V), device_id))
+    return w, V
+
+
 def one_body_gpu(basis, structure, op: int, device_id: int = 0) -> np.ndarray:
     """S (0), T (1) or V (2) computed on the GPU (OneBodyIntegral::
     overlap_integral / kinetic_integral / nuclear_integral, int1e.h)."""

@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .eri import ERIConfig, create, one_body, one_body_gpu, one_body_grad, one_body_grad_gpu
+from .eri import ERIConfig, create, eigh_gpu, one_body, one_body_gpu, one_body_grad, one_body_grad_gpu
 
 
 @dataclass
@@ -42,6 +42,8 @@ class SCFConfig:
     eri: ERIConfig = field(default_factory=ERIConfig)
     # S, T, V and their derivatives on the ERI engine's GPU (else on the host)
     one_body_on_gpu: bool = True
+    # diagonalise the orthogonalised Fock matrix on that GPU (rocSOLVER dsyevd)
+    eigen_on_gpu: bool = True
 
 
 def _one_body(basis, structure, op, cfg):
@@ -144,7 +146,12 @@ class RHF:
         self.build_stats = []
 
     def _diag(self, F):
-        e, Cp = np.linalg.eigh(self.X.T @ F @ self.X)
+        Fo = self.X.T @ F @ self.X
+        Fo = 0.5 * (Fo + Fo.T)
+        if self.cfg.eigen_on_gpu:
+            e, Cp = eigh_gpu(Fo, self.cfg.eri.device_id)
+        else:
+            e, Cp = np.linalg.eigh(Fo)
         C = self.X @ Cp
         return e, C
 
