@@ -151,6 +151,31 @@ def log_classes(run, steps, top=12):
             f"prim {npq:.3g}  {tf:6.2f} TFLOP/s  atomics {at:.3g}")
 
 
+def harness_timings(basis, mol, P, host=True):
+    """Seconds for the SCF harness pieces that run beside the Fock build on
+    the GPU: S + T + V, their three gradient contractions, and one
+    diagonalisation of an nbf x nbf symmetric matrix; with their host
+    counterparts (16-thread C++ integrals, numpy/LAPACK eigh) when host."""
+    import numpy as np
+    from qdk_chemistry_amd.eri import eigh_gpu, one_body, one_body_gpu, one_body_grad_gpu
+
+    def timed(f):
+        f()  # first call: tables, handles
+        t = time.perf_counter()
+        f()
+        return time.perf_counter() - t
+
+    A = 0.5 * (P + P.T)
+    out = dict(one_body_gpu_s=timed(lambda: [one_body_gpu(basis, mol, op) for op in range(3)]),
+               one_body_grad_gpu_s=timed(lambda: [one_body_grad_gpu(basis, mol, op, P) for op in range(3)]),
+               eigh_gpu_s=timed(lambda: eigh_gpu(A)))
+    if host:
+        out.update(one_body_host_s=timed(lambda: [one_body(basis, mol, op) for op in range(3)]),
+                   eigh_host_s=timed(lambda: np.linalg.eigh(A)))
+    log("harness: " + ", ".join(f"{k} {v:.3f}" for k, v in out.items()))
+    return out
+
+
 def cpu_baseline(basis, mol, P, stride):
     from oracle.oracle import OracleERI
     threads = min(16, os.cpu_count() or 1)
@@ -287,6 +312,8 @@ def main():
                      "calibration_copy_bytes": calib_bytes},
     }
     out["config"].update(result)
+    if "scf_fp64" in result:
+        out["config"]["harness"] = harness_timings(basis, mol, P, not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(basis, mol, P, args.cpu_stride)
         out["cpu_baseline"] = cb

@@ -240,6 +240,8 @@ struct Engine {
   DevBuf<double4> ppos_d;
   DevBuf<double> pk_d;
   DevBuf<double> pa_d;      // exponent of each primitive pair's first shell (gradients)
+  DevBuf<double> h2_d;      // 1/(2p) per primitive pair (p = exponent sum)
+  DevBuf<float> h2_32_d;
   DevBuf<int> sh_atom_d;
   DevBuf<double> grad_d;    // [2][3 * natoms]
   DevBuf<float4> ppos32_d;  // single-precision copies for the FP32 tasks
@@ -340,6 +342,16 @@ struct Engine {
     ra.boys = boys_d.p;
   }
 
+  // 1/(2p) of every primitive pair, in the order of the ppos table it goes with
+  void upload_h2(const std::vector<double4>& pp) {
+    std::vector<double> h(pp.size());
+    for (size_t i = 0; i < pp.size(); ++i) h[i] = 0.5 / pp[i].x;
+    h2_d.upload(h, stream);
+    h2_32_d.upload(to_float(h), stream);
+    ra.h2 = h2_d.p;
+    ra32.h2 = h2_32_d.p;
+  }
+
   void build_pairs() {
     // significant pairs (libint2_direct.cpp:83-151)
     struct RawPair {
@@ -401,6 +413,7 @@ struct Engine {
     pairs_d.upload(hdr, stream);
     ppos_d.upload(ppos, stream);
     pk_d.upload(pk, stream);
+    upload_h2(ppos);
     DevBuf<double> q2;
     q2.alloc(npair);
     HIP_CHECK(hipMemsetAsync(q2.p, 0, sizeof(double) * npair, stream));
@@ -524,6 +537,7 @@ struct Engine {
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
     pa_d.upload(pa2, stream);
+    upload_h2(ppos2);
     {
       std::vector<float4> p32(ppos2.size());
       for (size_t i = 0; i < ppos2.size(); ++i)

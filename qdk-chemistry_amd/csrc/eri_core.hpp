@@ -82,6 +82,7 @@ struct RysArgsT {
   R hx2[9][8];        // large-T limit: u_i = hx2/T
   R hw[9][8];         //               w_i = hw/sqrt(T)
   const R* boys;      // Boys Taylor table [BOYS_NG][BOYS_W]: F_m(k * BOYS_DT)
+  const R* h2;        // 1/(2p) per primitive pair (p = exponent sum), in ppos order
 };
 using RysArgs = RysArgsT<double>;
 using RysArgs32 = RysArgsT<float>;
@@ -348,7 +349,7 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
     const R Kk = pk[ik];
     const R q = Q.x;
     const R QCx = Q.y - Cx, QCy = Q.z - Cy, QCz = Q.w - Cz;
-    const R h2q = R(0.5) * frcp(q);
+    const R h2q = ra.h2[ik];
     for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
       const auto P = ppos[ib];
       const R Kb = pk[ib];
@@ -363,7 +364,7 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
       R u[C::NR], w[C::NR];
       rys_roots<C::NR>(T, ra, u, w);
       const R qr = q * rpq, pr = p * rpq;
-      const R h2p = R(0.5) * frcp(p);
+      const R h2p = ra.h2[ib];
 #pragma unroll 1
       for (int r = 0; r < C::NR; ++r) {
         const R ur = u[r];
