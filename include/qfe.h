@@ -131,6 +131,20 @@ int qfe_one_body_grad_gpu(const qfe_basis* basis, int op, const double* W, doubl
  * scf_algorithm.cpp:105-152, scf_impl.cpp:431-449, and its LAPACK one). */
 int qfe_eigh_gpu(int n, const double* A, double* w, double* V, int device_id);
 
+/* Density-fitted Coulomb matrix on GPU device_id.  Replaces the J of the
+ * reference's INCORE DF engine (ERI_DF, scf/src/eri/INCORE/
+ * incore_impl_df.cpp:53-200; eri_df_base.cpp): create computes the
+ * three-centre integrals (mu nu|P) over the auxiliary basis `aux` and the
+ * Cholesky factor of the metric (P|Q) once (orbital shells up to d,
+ * auxiliary shells up to g); build_j returns J = B V^-1 B^T vec(P) for a
+ * host density (nbf x nbf).  K is not fitted: the multiplexer pairs this J
+ * with the exchange of the Rys engine (eri_multiplexer.cpp:34-71). */
+typedef struct qfe_dfj qfe_dfj;
+int qfe_dfj_create(const qfe_basis* basis, const qfe_basis* aux, int device_id, qfe_dfj** out);
+int qfe_dfj_build_j(qfe_dfj* h, const double* P, double* J);
+int qfe_dfj_naux(const qfe_dfj* h);
+void qfe_dfj_destroy(qfe_dfj* h);
+
 /* Switch the precision policy of subsequent builds: mix != 0 evaluates
  * quartets with Q_ab Q_cd < threshold in FP32 (ERIConfig::is_mix_precision,
  * core/scf.h:71-93, which the reference declares but no shipped engine reads);

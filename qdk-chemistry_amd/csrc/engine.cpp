@@ -33,6 +33,7 @@
 #include "eri_core.hpp"
 #include "host_common.hpp"
 #include "kernels.hpp"
+#include "dfj.hpp"
 #include "onebody_gpu.hpp"
 #include "screen.hpp"
 
@@ -1009,6 +1010,62 @@ static rocblas_handle solver_handle(int device) {
   return cache.emplace(device, h).first->second;
 }
 
+static void one_body_device(int device_id);
+
+// Density-fitted J (the reference's INCORE DF engine, incore_impl_df.cpp):
+// B[P][mu][nu] = (mu nu|P) and the Cholesky factor of V[P][Q] = (P|Q) kept in
+// HBM; a build is X = B . vec(D), V c = X, J = B^T c (two GEMVs and a
+// triangular solve pair).
+struct DFJ {
+  int device = 0, nbf = 0, naux = 0;
+  DevBuf<double> B, V, d, x, j;
+  rocblas_handle blas = nullptr;
+
+  static DFBasisView view(const BasisHost& b, std::vector<DevBuf<int>>& ib, std::vector<DevBuf<double>>& db,
+                          int lmax, const char* what) {
+    std::vector<int> l(b.nsh), po(b.nsh + 1, 0), bf0(b.nsh);
+    std::vector<double> cen(3 * (size_t)b.nsh), ex, co;
+    for (int s = 0; s < b.nsh; ++s) {
+      const ShellHost& sh = b.shells[s];
+      if (sh.l > lmax) throw Unsupported(std::string("DF-J: ") + what + " shells above l = " + std::to_string(lmax));
+      l[s] = sh.l;
+      bf0[s] = b.bf0[s];
+      for (int k = 0; k < 3; ++k) cen[3 * s + k] = sh.O[k];
+      ex.insert(ex.end(), sh.a.begin(), sh.a.end());
+      co.insert(co.end(), sh.c.begin(), sh.c.end());
+      po[s + 1] = (int)ex.size();
+    }
+    ib.resize(3);
+    db.resize(3);
+    ib[0].upload(l, nullptr);
+    ib[1].upload(po, nullptr);
+    ib[2].upload(bf0, nullptr);
+    db[0].upload(cen, nullptr);
+    db[1].upload(ex, nullptr);
+    db[2].upload(co, nullptr);
+    return DFBasisView{ib[0].p, ib[1].p, ib[2].p, db[0].p, db[1].p, db[2].p, b.nsh, b.nbf, b.pure ? 1 : 0};
+  }
+
+  DFJ(const BasisHost& orb, const BasisHost& aux, int dev);
+
+  void build_j(const double* P, double* J) {
+    HIP_CHECK(hipSetDevice(device));
+    const size_t nn = (size_t)nbf * nbf;
+    HIP_CHECK(hipMemcpy(d.p, P, sizeof(double) * nn, hipMemcpyHostToDevice));
+    const double one = 1.0, zero = 0.0;
+    // B is row-major [naux][nbf^2], i.e. column-major nbf^2 x naux
+    if (rocblas_dgemv(blas, rocblas_operation_transpose, (rocblas_int)nn, naux, &one, B.p, (rocblas_int)nn, d.p, 1,
+                      &zero, x.p, 1) != rocblas_status_success)
+      throw DeviceError("DF-J: rocblas_dgemv failed");
+    if (rocsolver_dpotrs(blas, rocblas_fill_lower, naux, 1, V.p, naux, x.p, naux) != rocblas_status_success)
+      throw DeviceError("DF-J: rocsolver_dpotrs failed");
+    if (rocblas_dgemv(blas, rocblas_operation_none, (rocblas_int)nn, naux, &one, B.p, (rocblas_int)nn, x.p, 1, &zero,
+                      j.p, 1) != rocblas_status_success)
+      throw DeviceError("DF-J: rocblas_dgemv failed");
+    HIP_CHECK(hipMemcpy(J, j.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+  }
+};
+
 static void one_body_device(int device_id) {
   int ndev = 0;
   HIP_CHECK(hipGetDeviceCount(&ndev));
@@ -1230,6 +1287,69 @@ int qfe_eigh_gpu(int n, const double* A, double* w, double* V, int device_id) {
       for (int j = 0; j < n; ++j) V[(size_t)i * n + j] = cm[(size_t)j * n + i];
   });
 }
+
+}  // extern "C"
+
+namespace qfe {
+DFJ::DFJ(const BasisHost& orb, const BasisHost& aux, int dev) : device(dev), nbf(orb.nbf), naux(aux.nbf) {
+  one_body_device(dev);
+  if (nbf <= 0 || naux <= 0) throw std::invalid_argument("DF-J: empty basis");
+  const OneBodyTables& tb = one_body_tables(dev);
+  std::vector<DevBuf<int>> oi, xi;
+  std::vector<DevBuf<double>> od, xd;
+  const DFBasisView ov = view(orb, oi, od, 2, "orbital");
+  const DFBasisView xv = view(aux, xi, xd, 4, "auxiliary");
+  const size_t nn = (size_t)nbf * nbf;
+  B.alloc(nn * naux);
+  V.alloc((size_t)naux * naux);
+  d.alloc(nn);
+  j.alloc(nn);
+  x.alloc(naux);
+  launch_df_three_centre(ov, xv, tb.ra, tb.sph, B.p, nullptr);
+  HIP_CHECK(hipGetLastError());
+  launch_df_metric(xv, tb.ra, tb.sph, V.p, nullptr);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+  blas = solver_handle(dev);
+  DevBuf<rocblas_int> info;
+  info.alloc(1);
+  // V is symmetric: the column-major lower factor is the row-major upper one
+  if (rocsolver_dpotrf(blas, rocblas_fill_lower, naux, V.p, naux, info.p) != rocblas_status_success)
+    throw DeviceError("DF-J: rocsolver_dpotrf failed");
+  rocblas_int hinfo = 0;
+  HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
+  if (hinfo != 0) throw std::runtime_error("DF-J: the auxiliary metric is not positive definite");
+}
+}  // namespace qfe
+
+struct qfe_dfj {
+  std::unique_ptr<qfe::DFJ> e;
+};
+
+extern "C" {
+
+int qfe_dfj_create(const qfe_basis* basis, const qfe_basis* aux, int device_id, qfe_dfj** out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("null handle pointer");
+    *out = nullptr;
+    const qfe::BasisHost o = qfe::BasisHost::from_c(basis);
+    const qfe::BasisHost a = qfe::BasisHost::from_c(aux);
+    auto h = std::make_unique<qfe_dfj>();
+    h->e = std::make_unique<qfe::DFJ>(o, a, device_id);
+    *out = h.release();
+  });
+}
+
+int qfe_dfj_build_j(qfe_dfj* h, const double* P, double* J) {
+  return guarded([&] {
+    if (!h || !h->e || !P || !J) throw std::invalid_argument("null argument");
+    h->e->build_j(P, J);
+  });
+}
+
+int qfe_dfj_naux(const qfe_dfj* h) { return (h && h->e) ? h->e->naux : -1; }
+
+void qfe_dfj_destroy(qfe_dfj* h) { delete h; }
 
 int qfe_max_l(void) { return QFE_MAX_L; }
 

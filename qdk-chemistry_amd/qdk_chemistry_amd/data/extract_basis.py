@@ -1,5 +1,6 @@
 """Extract H/Li/C/N/O subsets (and Te/I of def2-SVP, with their ECP core
-electron counts) of the basis-set-exchange JSON files that
+electron counts) of the basis-set-exchange JSON files (orbital bases, and
+the def2-universal-jfit auxiliary basis of density-fitted J) that
 qdk-chemistry ships (cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/
 compressed/<name>.tar.gz, member basis/<name>.json) into data/basis/.
 
@@ -16,7 +17,7 @@ import sys
 import tarfile
 
 REF = "/root/reference/cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/compressed"
-NAMES = ["cc-pvdz", "cc-pvtz", "def2-svp", "def2-tzvp", "6-31g_st_", "sto-3g"]
+NAMES = ["cc-pvdz", "cc-pvtz", "def2-svp", "def2-tzvp", "6-31g_st_", "sto-3g", "def2-universal-jfit"]
 ELEMENTS = ["1", "3", "6", "7", "8"]
 EXTRA = {"def2-svp": ["52", "53"]}  # Te, I: the one-electron integral fixture's molecule
 

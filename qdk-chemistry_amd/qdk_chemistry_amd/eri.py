@@ -137,6 +137,43 @@ class RysERI:
         return {k: getattr(s, k) for k, _ in _lib.QfeStats._fields_}
 
 
+class DFJ:
+    """Density-fitted Coulomb engine on the GPU (the J of the reference's
+    INCORE DF engine, incore_impl_df.cpp): (mu nu|P) over the auxiliary
+    basis and the Cholesky-factored metric are built once; build_JK returns
+    J = B V^-1 B^T vec(P) and a zero K, for use as the J engine of an
+    ERIMultiplexer (eri_multiplexer.cpp:34-71)."""
+
+    def __init__(self, basis, aux_basis, structure, device_id: int = 0):
+        self.nbf = basis.nbf
+        self._b = _lib.BasisArrays(basis, structure)
+        self._a = _lib.BasisArrays(aux_basis, structure)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().qfe_dfj_create(ctypes.byref(self._b.c), ctypes.byref(self._a.c), device_id,
+                                             ctypes.byref(h)))
+        self.h = h
+        self.naux = int(_lib.lib().qfe_dfj_naux(h))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            _lib.lib().qfe_dfj_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        if P.shape != (self.nbf, self.nbf):
+            raise ValueError("DFJ.build_JK takes one nbf x nbf density")
+        J = np.empty_like(P)
+        _lib.check(_lib.lib().qfe_dfj_build_j(self.h, _lib.ptr(P), _lib.ptr(J)))
+        return J, np.zeros_like(P)
+
+
 class ERIMultiplexer:
     """Routes J and K to one engine, or J and K to two engines (J with omega 0),
     and gradients to a third when given; restates ERIMultiplexer

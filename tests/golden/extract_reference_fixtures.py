@@ -4,7 +4,8 @@ and its one-electron integral fixture into int1e_teh2hi.npz.
 Sources (data only): scf/tests/data/h2o.json and water10.json (RHF/HF/6-31g*/
 pure energy + gradient, atom-major), geometries from scf/tests/test_common.h
 (make_h2o, make_water10, make_TeH2_HI; angstrom with the local bohr factor
-0.52917721092); scf/tests/data/int1e.json (Psi4 S, T, V of TeH2-HI in
+0.52917721092); scf/tests/data/h2o_gauxc.json (RHF with density-fitted J);
+scf/tests/data/int1e.json (Psi4 S, T, V of TeH2-HI in
 def2-SVP, cartesian, unsorted shells, with the density `dm` and the three
 gradients sum dm dO/dR, natoms x 3).  Run in a checkout that holds the
 reference; the outputs are committed.
@@ -38,6 +39,14 @@ def main():
             grad = json.loads(grad)
         out[name] = dict(numbers=Z, coords_angstrom=R, basis="6-31g*", pure=True,
                          energy=float(rec["energy"]), gradient=[float(g) for g in grad])
+    # RHF with density-fitted J (def2-universal-jfit) and exact K, from the
+    # data file the reference's test reads for it (scf_test.cpp:22-25:
+    # "h2o_gauxc"; molecule make_h2o)
+    h2o = json.load(open(os.path.join(ROOT, "data", "h2o_gauxc.json")))
+    dfj = h2o["RHF-DFJ/def2-svp/def2-universal-jfit/pure"]
+    assert dfj["mol"] == "h2o" and dfj["dfj"] and dfj["aux_basis"] == "def2-universal-jfit"
+    out["h2o_dfj"] = dict(basis="def2-svp", aux_basis="def2-universal-jfit", pure=True,
+                          energy=float(dfj["energy"]))
     json.dump(out, open(os.path.join(HERE, "reference_fixtures.json"), "w"), indent=1)
     import numpy as np
     rec = json.load(open(os.path.join(ROOT, "data", "int1e.json")))
