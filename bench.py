@@ -176,6 +176,46 @@ def harness_timings(basis, mol, P, host=True):
     return out
 
 
+def mo_transform_timings(nwat, nt, host=True):
+    """Seconds for the first-quarter transform (nbf^3 x nt) and the
+    four-index MO transform (nt^4) of a water cluster in cc-pVDZ on the GPU
+    (dense AO tensor in HBM + rocBLAS GEMMs, output copied to the host),
+    and, when host, the oracle's full tensor plus numpy contractions on the
+    host cores for the same transform (a CPU baseline)."""
+    import numpy as np
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import MOERI, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(nwat, 4000 + nwat)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    nt = min(nt, b.nbf)
+    C = np.random.Generator(np.random.PCG64(nt)).uniform(-1.0, 1.0, (b.nbf, nt))
+    eri = create(b, mol)
+    mo = MOERI(eri)
+    eri.quarter_trans(C[:, :1])  # tables and handles
+    t = time.perf_counter()
+    q = eri.quarter_trans(C)
+    tq = time.perf_counter() - t
+    t = time.perf_counter()
+    m = mo.compute(b.nbf, nt, C, C, C, C)
+    tm = time.perf_counter() - t
+    out = dict(workload=f"(H2O){nwat} cc-pVDZ, nbf {b.nbf}, nt {nt}", quarter_trans_gpu_s=tq, mo_eri_gpu_s=tm)
+    if host:
+        from oracle.oracle import OracleERI, mo_eri
+        threads = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        T = OracleERI(b, mol, nthreads=threads).eri_full()
+        ref = mo_eri(T, C, C, C, C)
+        out["mo_eri_host_s"] = time.perf_counter() - t
+        out["host_threads"] = threads
+        out["max_abs_diff_vs_host"] = float(np.max(np.abs(ref - m)))
+        del T, ref
+    del q, m
+    log("mo transform: " + ", ".join(f"{k} {v:.3g}" if isinstance(v, float) else f"{k} {v}"
+                                      for k, v in out.items()))
+    return out
+
+
 def cpu_baseline(basis, mol, P, stride):
     from oracle.oracle import OracleERI
     threads = min(16, os.cpu_count() or 1)
@@ -200,6 +240,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=24)
+    ap.add_argument("--mo-waters", type=int, default=4, help="water cluster of the MO-transform timing (0: skip)")
+    ap.add_argument("--mo-nt", type=int, default=48)
+    ap.add_argument("--mo-only", action="store_true", help="only the MO-transform timing, one JSON line")
     ap.add_argument("--density", help="load the density from this .npy instead of running the SCF")
     ap.add_argument("--save-density", help="write the converged density to this .npy")
     args = ap.parse_args()
@@ -207,6 +250,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-device "
               f"in this build.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.mo_only:
+        print(json.dumps({"mo_transform": mo_transform_timings(args.mo_waters, args.mo_nt, not args.no_cpu)}),
+              flush=True)
+        return
 
     import numpy as np
     from qdk_chemistry_amd.basis import load_basis
@@ -314,6 +361,8 @@ def main():
     out["config"].update(result)
     if "scf_fp64" in result:
         out["config"]["harness"] = harness_timings(basis, mol, P, not args.no_cpu)
+    if args.mo_waters > 0:
+        out["config"]["mo_transform"] = mo_transform_timings(args.mo_waters, args.mo_nt, not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(basis, mol, P, args.cpu_stride)
         out["cpu_baseline"] = cb

@@ -145,6 +145,23 @@ int qfe_dfj_build_j(qfe_dfj* h, const double* P, double* J);
 int qfe_dfj_naux(const qfe_dfj* h);
 void qfe_dfj_destroy(qfe_dfj* h);
 
+/* First quarter of the MO transform, ERI::quarter_trans(nt, C, out)
+ * (eri.h:116; libint2_direct.cpp:913-1141, incore_impl.cpp:242-297):
+ * out[i][j][k][p] = sum_l (ij|kl) C[l][p], C row-major nbf x nt, out
+ * nbf^3 x nt with p fastest (host memory).  The dense AO tensor is computed
+ * in HBM (shells up to d) and contracted with one rocBLAS GEMM; nbf^4
+ * doubles must fit in free device memory (QFE_ERR_UNSUPPORTED otherwise).
+ * Requires 0 < nt <= nbf. */
+int qfe_quarter_trans(qfe_handle* h, unsigned long long nt, const double* C, double* out);
+
+/* Four-index MO transform, MOERI::compute(nb, nt, Ci, Cj, Ck, Cl, out)
+ * (moeri.cpp:28-200): out[w][x][y][z] = sum (ab|cd) Cl[a][w] Ck[b][x]
+ * Cj[c][y] Ci[d][z], each C row-major nb x nt, out nt^4 (host).  nb must be
+ * the basis size; nt > nb returns QFE_ERR_RUNTIME with the reference's
+ * message.  Four rocBLAS GEMMs over the dense tensor in HBM. */
+int qfe_mo_eri(qfe_handle* h, unsigned long long nb, unsigned long long nt, const double* Ci, const double* Cj,
+               const double* Ck, const double* Cl, double* out);
+
 /* Switch the precision policy of subsequent builds: mix != 0 evaluates
  * quartets with Q_ab Q_cd < threshold in FP32 (ERIConfig::is_mix_precision,
  * core/scf.h:71-93, which the reference declares but no shipped engine reads);

@@ -1147,13 +1147,10 @@ void oracle_census(const oracle_t *h, const double *P, double *nq, double *nprim
 
 /* INCORE-semantics J/K from the full (unscreened) tensor (incore_impl.cpp:159-197):
  * J_mn = sum_ls P_ls (mn|ls), K_mn = (alpha+beta) sum_ls P_ls (ml|ns).  Small N only. */
-int oracle_jk_incore(const oracle_t *h, const double *P, double *J, double *K, double alpha,
-                     double beta) {
+/* the full tensor T[i][j][k][l] = (ij|kl) over basis functions, every shell
+   quartet computed (no screening) */
+static void fill_full(const oracle_t *h, double *T) {
   const int n = h->nbf, nsh = h->nsh;
-  const size_t n2 = (size_t)n * n;
-  if ((double)n2 * n2 * 8.0 > 4e9) return -2;
-  double *T = (double *)malloc(sizeof(double) * n2 * n2);
-  if (!T) return -2;
   int lmax = 0;
   for (int s = 0; s < nsh; ++s)
     if (h->sh[s].l > lmax) lmax = h->sh[s].l;
@@ -1181,6 +1178,25 @@ int oracle_jk_incore(const oracle_t *h, const double *P, double *J, double *K, d
     free(buf);
     work_free(&w);
   }
+}
+
+/* the dense AO tensor, the input of the quarter transform
+   (incore_impl.cpp:242-297 contracts it with C; libint2_direct.cpp:913-1141
+   computes the same contraction quartet by quartet) */
+int oracle_eri_full(const oracle_t *h, double *T) {
+  if (!h || !T) return -1;
+  fill_full(h, T);
+  return 0;
+}
+
+int oracle_jk_incore(const oracle_t *h, const double *P, double *J, double *K, double alpha,
+                     double beta) {
+  const int n = h->nbf;
+  const size_t n2 = (size_t)n * n;
+  if ((double)n2 * n2 * 8.0 > 4e9) return -2;
+  double *T = (double *)malloc(sizeof(double) * n2 * n2);
+  if (!T) return -2;
+  fill_full(h, T);
   const double ks = alpha + beta;
 #pragma omp parallel for num_threads(h->nthreads)
   for (int m = 0; m < n; ++m)

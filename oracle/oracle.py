@@ -52,6 +52,8 @@ def lib():
                                       ctypes.c_double, ctypes.c_int, _dp]
         L.oracle_census.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.oracle_jk_incore.restype = ctypes.c_int
+        L.oracle_eri_full.restype = ctypes.c_int
+        L.oracle_eri_full.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_jk_incore.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.oracle_gradients.restype = ctypes.c_int
         L.oracle_gradients.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double,
@@ -169,6 +171,14 @@ class OracleERI:
             raise MemoryError("system too large for the full-tensor oracle")
         return J, K
 
+    def eri_full(self):
+        """The dense AO tensor T[i, j, k, l] = (ij|kl), every quartet computed."""
+        n = self.nbf
+        T = np.zeros((n, n, n, n))
+        if lib().oracle_eri_full(self.h, _p(T)) != 0:
+            raise RuntimeError("oracle_eri_full failed")
+        return T
+
     def get_gradients(self, P, alpha=1.0, beta=0.0, mode=1, class_key=-1, centre=-1):
         """(dJ, dK), each 3*natoms xyz-major (libint2_direct.cpp:824); class_key
         (1000 la + 100 lb + 10 lc + ld) restricts the sum to one ERI class,
@@ -187,6 +197,35 @@ class OracleERI:
         if rc != 0:
             raise RuntimeError("Density matrix contains NaN/Inf values.")
         return dJ, dK
+
+
+def quarter_trans(T, C):
+    """out[i, j, k, p] = sum_l (ij|kl) C[l, p]: the first quarter of the MO
+    transform, nbf^3 x nt with p fastest (ERI::quarter_trans, eri.h:116; the
+    in-core form incore_impl.cpp:286-297 is this one GEMM over the dense
+    tensor; libint2_direct.cpp:913-1141 accumulates it quartet by quartet)."""
+    n = T.shape[0]
+    return (T.reshape(n * n * n, n) @ C).reshape(n, n, n, C.shape[1])
+
+
+def mo_eri(T, Ci, Cj, Ck, Cl):
+    """The four-quarter transform of MOERI::compute (moeri.cpp:28-200, CPU
+    path :118-198): tmp1 = quarter_trans(Ci); the 2nd quarter contracts the
+    third AO index with Cj (:124-133), the 3rd the second with Ck (:135-144),
+    the 4th the first with Cl (:146-151).  Row-major result
+    out[w, x, y, z] = sum (ab|cd) Cl[a, w] Ck[b, x] Cj[c, y] Ci[d, z]."""
+    t1 = quarter_trans(T, Ci)                           # [a, b, c, z]
+    t2 = np.einsum("abcz,cy->abyz", t1, Cj, optimize=True)
+    t3 = np.einsum("abyz,bx->axyz", t2, Ck, optimize=True)
+    return np.einsum("axyz,aw->wxyz", t3, Cl, optimize=True)
+
+
+def rmp2_correlation(ovov, eps_occ, eps_vir):
+    """Closed-shell MP2 correlation energy from (ia|jb) = ovov[i, a, j, b]:
+    sum (ia|jb) [2 (ia|jb) - (ib|ja)] / (e_i + e_j - e_a - e_b)."""
+    d = (eps_occ[:, None, None, None] - eps_vir[None, :, None, None]
+         + eps_occ[None, None, :, None] - eps_vir[None, None, None, :])
+    return float(np.sum(ovov * (2.0 * ovov - ovov.transpose(0, 3, 2, 1)) / d))
 
 
 def orthogonalizer(S, lindep=-1.0):

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -34,6 +36,7 @@
 #include "host_common.hpp"
 #include "kernels.hpp"
 #include "dfj.hpp"
+#include "eri_dense.hpp"
 #include "onebody_gpu.hpp"
 #include "screen.hpp"
 
@@ -1027,7 +1030,7 @@ struct DFJ {
     std::vector<double> cen(3 * (size_t)b.nsh), ex, co;
     for (int s = 0; s < b.nsh; ++s) {
       const ShellHost& sh = b.shells[s];
-      if (sh.l > lmax) throw Unsupported(std::string("DF-J: ") + what + " shells above l = " + std::to_string(lmax));
+      if (sh.l > lmax) throw Unsupported(std::string(what) + " shells above l = " + std::to_string(lmax));
       l[s] = sh.l;
       bf0[s] = b.bf0[s];
       for (int k = 0; k < 3; ++k) cen[3 * s + k] = sh.O[k];
@@ -1297,8 +1300,8 @@ DFJ::DFJ(const BasisHost& orb, const BasisHost& aux, int dev) : device(dev), nbf
   const OneBodyTables& tb = one_body_tables(dev);
   std::vector<DevBuf<int>> oi, xi;
   std::vector<DevBuf<double>> od, xd;
-  const DFBasisView ov = view(orb, oi, od, 2, "orbital");
-  const DFBasisView xv = view(aux, xi, xd, 4, "auxiliary");
+  const DFBasisView ov = view(orb, oi, od, 2, "DF-J: orbital");
+  const DFBasisView xv = view(aux, xi, xd, 4, "DF-J: auxiliary");
   const size_t nn = (size_t)nbf * nbf;
   B.alloc(nn * naux);
   V.alloc((size_t)naux * naux);
@@ -1319,6 +1322,90 @@ DFJ::DFJ(const BasisHost& orb, const BasisHost& aux, int dev) : device(dev), nbf
   rocblas_int hinfo = 0;
   HIP_CHECK(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
   if (hinfo != 0) throw std::runtime_error("DF-J: the auxiliary metric is not positive definite");
+}
+
+// ------------------------------------------------------------ MO transforms
+// The dense AO tensor T[a][b][c][d] = (ab|cd) in HBM (nbf^4 doubles, shells
+// up to d), for the quarter transform (ERI::quarter_trans, eri.h:116) and
+// MOERI::compute (moeri.cpp:28-200).  Small systems only: nbf = 200 is
+// 12.8 GB.
+static void dense_tensor(const BasisHost& b, int dev, DevBuf<double>& T, size_t extra_doubles) {
+  one_body_device(dev);
+  const size_t n = (size_t)b.nbf, n4 = n * n * n * n;
+  size_t free_b = 0, total_b = 0;
+  HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if ((double)(n4 + extra_doubles) * sizeof(double) > 0.9 * (double)free_b)
+    throw Unsupported("MO transform: the dense AO tensor (" + std::to_string(n4 * sizeof(double) >> 20) +
+                      " MiB) and its work buffer do not fit in free device memory");
+  const OneBodyTables& tb = one_body_tables(dev);
+  std::vector<DevBuf<int>> ib;
+  std::vector<DevBuf<double>> db;
+  const DFBasisView v = DFJ::view(b, ib, db, 2, "MO transform: orbital");
+  T.alloc(n4);
+  launch_eri_dense(v, tb.ra, tb.sph, T.p, nullptr);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+static rocblas_int blas_dim(size_t v) {
+  if (v > (size_t)std::numeric_limits<rocblas_int>::max())
+    throw Unsupported("MO transform: a GEMM dimension exceeds the rocBLAS integer range");
+  return (rocblas_int)v;
+}
+
+// column-major C = op(A) op(B), alpha 1, beta 0
+static void dgemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, size_t m, size_t n, size_t k,
+                  const double* A, size_t lda, const double* B, size_t ldb, double* C, size_t ldc) {
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemm(h, ta, tb, blas_dim(m), blas_dim(n), blas_dim(k), &one, A, blas_dim(lda), B, blas_dim(ldb),
+                    &zero, C, blas_dim(ldc)) != rocblas_status_success)
+    throw DeviceError("MO transform: rocblas_dgemm failed");
+}
+
+// out[a][b][c][p] = sum_d T[a][b][c][d] C[d][p]  (C row-major nbf x nt):
+// column-major out (nt x nbf^3) = C (nt x nbf) . T (nbf x nbf^3)
+static void quarter_trans(const BasisHost& b, int dev, size_t nt, const double* C, double* out) {
+  const size_t n = (size_t)b.nbf, n3 = n * n * n;
+  if (nt == 0 || nt > n) throw std::invalid_argument("quarter_trans: need 0 < nt <= nbf");
+  DevBuf<double> T, c, o;
+  dense_tensor(b, dev, T, nt * n3 + nt * n);
+  c.alloc(n * nt);
+  o.alloc(nt * n3);
+  HIP_CHECK(hipMemcpy(c.p, C, sizeof(double) * n * nt, hipMemcpyHostToDevice));
+  rocblas_handle h = solver_handle(dev);
+  dgemm(h, rocblas_operation_none, rocblas_operation_none, nt, n3, n, c.p, nt, T.p, n, o.p, nt);
+  HIP_CHECK(hipMemcpy(out, o.p, sizeof(double) * nt * n3, hipMemcpyDeviceToHost));
+}
+
+// out[w][x][y][z] = sum (ab|cd) Cl[a][w] Ck[b][x] Cj[c][y] Ci[d][z], the
+// result of MOERI::compute.  Each quarter contracts the last index of a
+// row-major X[m][d] and moves the new index to the front,
+// Y[t][m] = sum_d X[m][d] C[d][t], one GEMM: column-major Y (M x nt) =
+// X^T (M x nbf) . C^T (nbf x nt).  The four steps take T[a][b][c][d] to
+// [z][a][b][c], [y][z][a][b], [x][y][z][a] and [w][x][y][z].  Two HBM
+// buffers: T (then the second and fourth results) and one of nt nbf^3.
+static void mo_transform(const BasisHost& b, int dev, size_t nt, const double* const Cs[4], double* out) {
+  const size_t n = (size_t)b.nbf;
+  if (nt == 0 || nt > n) throw std::invalid_argument("MO transform: need 0 < nt <= nbf");
+  const size_t n3 = n * n * n;
+  DevBuf<double> T, W, c[4];
+  dense_tensor(b, dev, T, nt * n3 + 4 * nt * n);
+  W.alloc(nt * n3);
+  for (int q = 0; q < 4; ++q) {
+    c[q].alloc(n * nt);
+    HIP_CHECK(hipMemcpy(c[q].p, Cs[q], sizeof(double) * n * nt, hipMemcpyHostToDevice));
+  }
+  rocblas_handle h = solver_handle(dev);
+  const rocblas_operation t = rocblas_operation_transpose;
+  size_t M = n3;  // rows of X: the product of the three leading extents
+  double* src = T.p;
+  double* dst = W.p;
+  for (int q = 0; q < 4; ++q) {  // Ci, Cj, Ck, Cl
+    dgemm(h, t, t, M, nt, n, src, n, c[q].p, nt, dst, M);
+    M = M / n * nt;
+    std::swap(src, dst);
+  }
+  HIP_CHECK(hipMemcpy(out, src, sizeof(double) * nt * nt * nt * nt, hipMemcpyDeviceToHost));
 }
 }  // namespace qfe
 
@@ -1348,6 +1435,24 @@ int qfe_dfj_build_j(qfe_dfj* h, const double* P, double* J) {
 }
 
 int qfe_dfj_naux(const qfe_dfj* h) { return (h && h->e) ? h->e->naux : -1; }
+
+int qfe_quarter_trans(qfe_handle* h, unsigned long long nt, const double* C, double* out) {
+  return guarded([&] {
+    if (!h || !C || !out) throw std::invalid_argument("quarter_trans: null argument");
+    qfe::quarter_trans(h->e.basis, h->e.device, (size_t)nt, C, out);
+  });
+}
+
+int qfe_mo_eri(qfe_handle* h, unsigned long long nb, unsigned long long nt, const double* Ci, const double* Cj,
+               const double* Ck, const double* Cl, double* out) {
+  return guarded([&] {
+    if (!h || !Ci || !Cj || !Ck || !Cl || !out) throw std::invalid_argument("mo_eri: null argument");
+    if (nb != (unsigned long long)h->e.nbf) throw std::invalid_argument("mo_eri: nb differs from the basis size");
+    if (nt > nb) throw std::runtime_error("MOERI does not support num_molecular_orbitals > num_atomic_orbitals");
+    const double* Cs[4] = {Ci, Cj, Ck, Cl};
+    qfe::mo_transform(h->e.basis, h->e.device, (size_t)nt, Cs, out);
+  });
+}
 
 void qfe_dfj_destroy(qfe_dfj* h) { delete h; }
 

@@ -1,0 +1,16 @@
+// The dense AO two-electron tensor for the quarter and MO transforms
+// (kernel in eri_dense.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dfj.hpp"
+#include "eri_core.hpp"
+
+namespace qfe {
+
+// eri[i][j][k][l] = (ij|kl), nbf^4 doubles (device, row-major), every shell
+// quartet computed; shells up to d; sph = the solid-harmonic table kSph
+void launch_eri_dense(const DFBasisView& b, const RysArgs& ra, const double* sph, double* eri, hipStream_t s);
+
+}  // namespace qfe

@@ -114,6 +114,19 @@ class RysERI:
         _lib.check(_lib.lib().qfe_gradients(self.h, _lib.ptr(P), _lib.ptr(dJ), _lib.ptr(dK), alpha, beta, omega))
         return dJ, dK
 
+    def quarter_trans(self, C):
+        """out[i, j, k, p] = sum_l (ij|kl) C[l, p] for C (nbf x nt): the first
+        quarter of the MO transform (ERI::quarter_trans, eri.h:116), on the
+        GPU over the dense AO tensor in HBM."""
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        n = self.basis.nbf
+        if C.ndim != 2 or C.shape[0] != n:
+            raise ValueError("quarter_trans takes C of shape (nbf, nt)")
+        nt = C.shape[1]
+        out = np.empty((n, n, n, nt))
+        _lib.check(_lib.lib().qfe_quarter_trans(self.h, nt, _lib.ptr(C), _lib.ptr(out)))
+        return out
+
     def set_precision(self, mix: bool, threshold: float | None = None):
         """Mixed (FP32 for Q_ab Q_cd < threshold) or all-FP64 builds from now on."""
         thr = self.cfg.mix_precision_threshold if threshold is None else threshold
@@ -172,6 +185,25 @@ class DFJ:
         J = np.empty_like(P)
         _lib.check(_lib.lib().qfe_dfj_build_j(self.h, _lib.ptr(P), _lib.ptr(J)))
         return J, np.zeros_like(P)
+
+
+class MOERI:
+    """Four-index MO transform over an engine (MOERI, moeri.cpp:28-200):
+    compute(nb, nt, Ci, Cj, Ck, Cl) returns out[w, x, y, z] =
+    sum (ab|cd) Cl[a, w] Ck[b, x] Cj[c, y] Ci[d, z], each C of shape (nb, nt);
+    with one C for all four, the MO integrals (pq|rs)."""
+
+    def __init__(self, eri: RysERI):
+        self.eri = eri
+
+    def compute(self, nb, nt, Ci, Cj, Ck, Cl):
+        Cs = [np.ascontiguousarray(C, dtype=np.float64) for C in (Ci, Cj, Ck, Cl)]
+        for C in Cs:
+            if C.shape != (nb, nt):
+                raise ValueError("MOERI.compute takes four C of shape (nb, nt)")
+        out = np.empty((nt, nt, nt, nt))
+        _lib.check(_lib.lib().qfe_mo_eri(self.eri.h, nb, nt, *[_lib.ptr(C) for C in Cs], _lib.ptr(out)))
+        return out
 
 
 class ERIMultiplexer:

@@ -5,6 +5,9 @@ Sources (data only): scf/tests/data/h2o.json and water10.json (RHF/HF/6-31g*/
 pure energy + gradient, atom-major), geometries from scf/tests/test_common.h
 (make_h2o, make_water10, make_TeH2_HI; angstrom with the local bohr factor
 0.52917721092); scf/tests/data/h2o_gauxc.json (RHF with density-fitted J);
+cpp/tests/test_mp2.cpp (RMP2 correlation energy of singlet O2 in
+cc-pVDZ at the test's default bond length, a PySCF value, with the
+tolerance testing::mp2_tolerance of cpp/tests/ut_common.hpp);
 scf/tests/data/int1e.json (Psi4 S, T, V of TeH2-HI in
 def2-SVP, cartesian, unsorted shells, with the density `dm` and the three
 gradients sum dm dO/dR, natoms x 3).  Run in a checkout that holds the
@@ -15,6 +18,7 @@ import os
 import re
 
 ROOT = "/root/reference/cpp/src/qdk/chemistry/algorithms/microsoft/scf/tests"
+TESTS = "/root/reference/cpp/tests"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -47,6 +51,15 @@ def main():
     assert dfj["mol"] == "h2o" and dfj["dfj"] and dfj["aux_basis"] == "def2-universal-jfit"
     out["h2o_dfj"] = dict(basis="def2-svp", aux_basis="def2-universal-jfit", pure=True,
                           energy=float(dfj["energy"]))
+    # RMP2 of singlet O2 (create_o2_hf_setup(1): O at 0 and at bond_length
+    # bohr on x, cc-pvdz; test RMP2Energies_CCPVDZ)
+    mp2 = open(os.path.join(TESTS, "test_mp2.cpp")).read()
+    bond = float(re.search(r"double bond_length = ([\d.]+)", mp2).group(1))
+    corr = float(re.search(r"pyscf_rmp2_corr_cc_pvdz = (-?[\d.]+)", mp2).group(1))
+    tol = float(re.search(r"mp2_tolerance = ([\d.e+-]+);",
+                          open(os.path.join(TESTS, "ut_common.hpp")).read()).group(1))
+    out["o2_rmp2"] = dict(numbers=[8, 8], coords_bohr=[[0.0, 0.0, 0.0], [bond, 0.0, 0.0]], basis="cc-pvdz",
+                          pure=True, correlation_energy=corr, tolerance=tol)
     json.dump(out, open(os.path.join(HERE, "reference_fixtures.json"), "w"), indent=1)
     import numpy as np
     rec = json.load(open(os.path.join(ROOT, "data", "int1e.json")))
