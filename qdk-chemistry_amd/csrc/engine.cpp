@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <limits>
+#include <tuple>
 #include <utility>
 #include <cmath>
 #include <cstdio>
@@ -1341,8 +1342,24 @@ static void dense_tensor(const BasisHost& b, int dev, DevBuf<double>& T, size_t 
   std::vector<DevBuf<int>> ib;
   std::vector<DevBuf<double>> db;
   const DFBasisView v = DFJ::view(b, ib, db, 2, "MO transform: orbital");
+  // every unordered shell pair once, the higher l first, grouped by class
+  // and contraction length so that a wave's quartets mostly share them
+  std::vector<int2> pr;
+  pr.reserve((size_t)b.nsh * (b.nsh + 1) / 2);
+  for (int s1 = 0; s1 < b.nsh; ++s1)
+    for (int s2 = 0; s2 <= s1; ++s2) {
+      const bool swap = b.shells[s2].l > b.shells[s1].l;
+      pr.push_back(int2{swap ? s2 : s1, swap ? s1 : s2});
+    }
+  auto key = [&](const int2& p) {
+    const ShellHost &x = b.shells[p.x], &y = b.shells[p.y];
+    return std::make_tuple(x.l, y.l, x.a.size() * y.a.size());
+  };
+  std::stable_sort(pr.begin(), pr.end(), [&](const int2& p, const int2& q) { return key(p) < key(q); });
+  DevBuf<int2> pairs;
+  pairs.upload(pr, nullptr);
   T.alloc(n4);
-  launch_eri_dense(v, tb.ra, tb.sph, T.p, nullptr);
+  launch_eri_dense(v, tb.ra, tb.sph, pairs.p, T.p, nullptr);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipDeviceSynchronize());
 }

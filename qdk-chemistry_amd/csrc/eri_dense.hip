@@ -79,6 +79,13 @@ __device__ void dn_cart(const Sh& a, const Sh& b, const Sh& c, const Sh& d, cons
     CD2 += CD[x] * CD[x];
   }
   const int nroots = (lab + lcd) / 2 + 1;
+  // cartesian exponents of every component, decoded once per quartet
+  int ex[4][3][kNC];
+  {
+    const int ls[4] = {la, lb, lc, ld};
+    for (int q = 0; q < 4; ++q)
+      for (int c = 0; c < dn_ncart(ls[q]); ++c) dn_cart_xyz(ls[q], c, ex[q][0][c], ex[q][1][c], ex[q][2][c]);
+  }
   for (int ia = 0; ia < a.n; ++ia)
     for (int ib = 0; ib < b.n; ++ib) {
       const double p = a.e[ia] + b.e[ib];
@@ -144,24 +151,14 @@ __device__ void dn_cart(const Sh& a, const Sh& b, const Sh& c, const Sh& d, cons
                     for (int m = 0; m <= ld; ++m) I[x][i][j][k][m] = t[k][m];
                 }
             }
-            for (int ca = 0; ca < na; ++ca) {
-              int ax, ay, az;
-              dn_cart_xyz(la, ca, ax, ay, az);
-              for (int cb = 0; cb < nb; ++cb) {
-                int bx, by, bz;
-                dn_cart_xyz(lb, cb, bx, by, bz);
-                for (int cc = 0; cc < nc; ++cc) {
-                  int cx, cy, cz;
-                  dn_cart_xyz(lc, cc, cx, cy, cz);
-                  for (int cd = 0; cd < nd; ++cd) {
-                    int dx, dy, dz;
-                    dn_cart_xyz(ld, cd, dx, dy, dz);
+            for (int ca = 0; ca < na; ++ca)
+              for (int cb = 0; cb < nb; ++cb)
+                for (int cc = 0; cc < nc; ++cc)
+                  for (int cd = 0; cd < nd; ++cd)
                     out[((ca * nb + cb) * nc + cc) * nd + cd] +=
-                        I[0][ax][bx][cx][dx] * I[1][ay][by][cy][dy] * I[2][az][bz][cz][dz];
-                  }
-                }
-              }
-            }
+                        I[0][ex[0][0][ca]][ex[1][0][cb]][ex[2][0][cc]][ex[3][0][cd]] *
+                        I[1][ex[0][1][ca]][ex[1][1][cb]][ex[2][1][cc]][ex[3][1][cd]] *
+                        I[2][ex[0][2][ca]][ex[1][2][cb]][ex[2][2][cc]][ex[3][2][cd]];
           }
         }
     }
@@ -203,20 +200,22 @@ __device__ void dn_tri(long long t, int& r, int& c) {
   c = (int)(t - (long long)x * (x + 1) / 2);
 }
 
-// One thread per canonical shell quartet (s1 >= s2, s3 >= s4, pair 12 >=
-// pair 34): the block is computed once and stored at its eight
+// One thread per unordered pair of shell pairs (p12 >= p34 over the list
+// `pairs`, which holds every unordered shell pair once, sorted by class and
+// contraction length so that a wave's quartets mostly share their class):
+// the block is computed once and stored at its eight
 // permutation-equivalent places (ab|cd) = (ba|cd) = (ab|dc) = (cd|ab) ...;
 // where shells coincide two of the stores hit the same element with the same
 // value.
 __global__ __launch_bounds__(256) void k_eri_dense(DFBasisView b, RysArgs ra, const double* __restrict__ sph,
-                                                   double* __restrict__ eri) {
+                                                   const int2* __restrict__ pairs, double* __restrict__ eri) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long np = (long long)b.nsh * (b.nsh + 1) / 2;
   if (t >= np * (np + 1) / 2) return;
-  int p12, p34, s1, s2, s3, s4;
+  int p12, p34;
   dn_tri(t, p12, p34);
-  dn_tri(p12, s1, s2);
-  dn_tri(p34, s3, s4);
+  const int2 u = pairs[p12], v = pairs[p34];
+  const int s1 = u.x, s2 = u.y, s3 = v.x, s4 = v.y;
   const Sh A = dn_shell(b, s1), B = dn_shell(b, s2), C = dn_shell(b, s3), D = dn_shell(b, s4);
   double x0[kBlk], x1[kBlk];
   dn_cart(A, B, C, D, ra, x0);
@@ -254,11 +253,12 @@ __global__ __launch_bounds__(256) void k_eri_dense(DFBasisView b, RysArgs ra, co
     }
 }
 
-void launch_eri_dense(const DFBasisView& b, const RysArgs& ra, const double* sph, double* eri, hipStream_t s) {
+void launch_eri_dense(const DFBasisView& b, const RysArgs& ra, const double* sph, const int2* pairs, double* eri,
+                      hipStream_t s) {
   const long long np = (long long)b.nsh * (b.nsh + 1) / 2;
   const long long n = np * (np + 1) / 2;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_eri_dense, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, ra, sph, eri);
+  hipLaunchKernelGGL(k_eri_dense, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, ra, sph, pairs, eri);
 }
 
 }  // namespace qfe
