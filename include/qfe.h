@@ -162,6 +162,34 @@ int qfe_quarter_trans(qfe_handle* h, unsigned long long nt, const double* C, dou
 int qfe_mo_eri(qfe_handle* h, unsigned long long nb, unsigned long long nt, const double* Ci, const double* Cj,
                const double* Ck, const double* Cl, double* out);
 
+/* Device-resident RHF iterations: the dense linear algebra of
+ * SCFImpl::iterate_ (scf_impl.cpp:555-621) and the DIIS of
+ * scf_algorithm.cpp:401-435 kept in the engine's HBM, replacing the host
+ * GEMMs and the per-iteration P/J/K copies (the reference's optional device
+ * path is the cuSOLVER eigensolve, scf_algorithm.cpp:105-152).
+ *   create: H, S (nbf x nbf), X (nbf x norb, row-major, the orthogonaliser),
+ *           nocc doubly occupied orbitals, E_nuc, DIIS depth.
+ *   set_density: the guess P (host).
+ *   step: J/K of P (full_build != 0: F = H + J - 1/2 K) or of P - P_last
+ *         (F += J - 1/2 K); returns E = E_nuc + 1/2 Tr(PH) + 1/2 Tr(PF), the
+ *         orbital gradient max|FPS - SPF| / nbf and ||P - P_prev||_F / nbf.
+ *   update: DIIS extrapolation over the stored (F, FPS - SPF), X^T F X,
+ *           rocSOLVER dsyevd, P = 2 C_occ C_occ^T.
+ *   finish: a full rebuild, F, E and the eigenvectors of that F.
+ *   get: what 0 = P, 1 = F (nbf x nbf), 2 = C (nbf x norb, row-major),
+ *        3 = orbital energies (norb).
+ * The context uses the engine's stream and device buffers: do not run other
+ * builds on the same handle between step and update. */
+typedef struct qfe_scf qfe_scf;
+int qfe_scf_create(qfe_handle* h, const double* H, const double* S, const double* X, int norb, int nocc,
+                   double enuc, int diis_size, qfe_scf** out);
+int qfe_scf_set_density(qfe_scf* s, const double* P);
+int qfe_scf_step(qfe_scf* s, int full_build, double* energy, double* orbital_gradient, double* density_rms);
+int qfe_scf_update(qfe_scf* s);
+int qfe_scf_finish(qfe_scf* s, double* energy);
+int qfe_scf_get(qfe_scf* s, int what, double* out);
+void qfe_scf_destroy(qfe_scf* s);
+
 /* Switch the precision policy of subsequent builds: mix != 0 evaluates
  * quartets with Q_ab Q_cd < threshold in FP32 (ERIConfig::is_mix_precision,
  * core/scf.h:71-93, which the reference declares but no shipped engine reads);

@@ -1424,10 +1424,262 @@ static void mo_transform(const BasisHost& b, int dev, size_t nt, const double* c
   }
   HIP_CHECK(hipMemcpy(out, src, sizeof(double) * nt * nt * nt * nt, hipMemcpyDeviceToHost));
 }
+
+// ------------------------------------------------------------ device-resident RHF iterations
+// The dense linear algebra of one RHF iteration kept in HBM (SCFImpl::
+// iterate_, scf_impl.cpp:555-621, with the DIIS of scf_algorithm.cpp:
+// 401-435): F = F + J - 1/2 K from a (difference) density, the energy
+// E_nuc + 1/2 Tr(P H) + 1/2 Tr(P F), the orbital gradient FPS - SPF, the DIIS
+// extrapolation, X^T F X, the eigensolve and P = 2 C_occ C_occ^T, all on the
+// engine's stream with one rocBLAS handle; only scalars and the small DIIS
+// system (at most diis_size + 1 square) cross to the host.  Matrices are
+// held column-major; S, H, F and P are symmetric, X (nbf x norb) is
+// transposed on upload.  The error matrix is stored as SPF - FPS, the
+// transpose of the row-major FPS - SPF: DIIS uses only its inner products
+// and its largest magnitude, which do not see the sign.
+struct SCFDevice {
+  Engine& e;
+  int n = 0, m = 0, nocc = 0, diis_size = 10;
+  double enuc = 0.0;
+  rocblas_handle blas = nullptr;
+  DevBuf<double> H, S, X, F, P, Plast, Pd, Pprev, T1, T2, Fo, C, w, ework, err;
+  DevBuf<rocblas_int> info;
+  std::vector<DevBuf<double>> Fh, Eh;  // DIIS ring of Fock and error matrices
+  std::vector<int> ring;               // ring slots, oldest first
+  std::vector<double> Bm;              // inner products of the stored errors [slot][slot]
+
+  SCFDevice(Engine& eng, const double* Hh, const double* Sh, const double* Xh, int norb, int nocc_, double enuc_,
+            int diis)
+      : e(eng), n(eng.nbf), m(norb), nocc(nocc_), diis_size(diis), enuc(enuc_) {
+    if (m <= 0 || m > n) throw std::invalid_argument("SCF: need 0 < norb <= nbf");
+    if (nocc <= 0 || nocc > m) throw std::invalid_argument("SCF: need 0 < nocc <= norb");
+    if (diis_size < 1) throw std::invalid_argument("SCF: diis_size must be positive");
+    HIP_CHECK(hipSetDevice(e.device));
+    if (rocblas_create_handle(&blas) != rocblas_status_success) throw DeviceError("rocblas_create_handle failed");
+    rocblas_set_stream(blas, e.stream);
+    const size_t nn = (size_t)n * n;
+    for (DevBuf<double>* b : {&H, &S, &F, &P, &Plast, &Pd, &Pprev, &T2, &err}) b->alloc(nn);
+    X.alloc((size_t)n * m);
+    T1.alloc((size_t)n * m);
+    C.alloc((size_t)n * m);
+    Fo.alloc((size_t)m * m);
+    w.alloc(m);
+    ework.alloc(m);
+    info.alloc(1);
+    e.P_d.alloc(nn);
+    e.J_d.alloc(nn);
+    e.K_d.alloc(nn);
+    std::vector<double> xc((size_t)n * m);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < m; ++j) xc[(size_t)j * n + i] = Xh[(size_t)i * m + j];
+    HIP_CHECK(hipMemcpyAsync(H.p, Hh, nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(S.p, Sh, nn * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemcpyAsync(X.p, xc.data(), xc.size() * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipMemsetAsync(Pprev.p, 0, nn * sizeof(double), e.stream));
+    HIP_CHECK(hipMemsetAsync(Plast.p, 0, nn * sizeof(double), e.stream));
+    HIP_CHECK(hipStreamSynchronize(e.stream));
+    Fh.resize(diis_size);
+    Eh.resize(diis_size);
+    Bm.assign((size_t)diis_size * diis_size, 0.0);
+  }
+  ~SCFDevice() {
+    if (blas) rocblas_destroy_handle(blas);
+  }
+
+  static void ok(rocblas_status st, const char* what) {
+    if (st != rocblas_status_success) throw DeviceError(std::string("SCF: ") + what + " failed");
+  }
+  // C = a A + b op(B), n x n
+  void geam(double a, const double* A, double b, const double* B, bool tb, double* Cm) {
+    ok(rocblas_dgeam(blas, rocblas_operation_none, tb ? rocblas_operation_transpose : rocblas_operation_none, n, n,
+                     &a, A, n, &b, B, n, Cm, n),
+       "dgeam");
+  }
+  double dot(const double* a, const double* b) {
+    double r = 0.0;
+    ok(rocblas_ddot(blas, n * n, a, 1, b, 1, &r), "ddot");
+    return r;
+  }
+  void copy(const double* src, double* dst, size_t count) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToDevice, e.stream));
+  }
+
+  void set_density(const double* Ph) {
+    HIP_CHECK(hipMemcpyAsync(P.p, Ph, (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    HIP_CHECK(hipStreamSynchronize(e.stream));
+  }
+
+  // J/K of P (full) or of P - P_last (incremental), F, E, orbital gradient
+  // and density change; out = {E, og, drms}
+  void step(bool full, double out[3]) {
+    const size_t nn = (size_t)n * n;
+    const double one = 1.0, zero = 0.0;
+    if (full) {
+      copy(P.p, e.P_d.p, nn);
+      copy(H.p, F.p, nn);
+    } else {
+      geam(1.0, P.p, -1.0, Plast.p, false, e.P_d.p);
+    }
+    copy(P.p, Plast.p, nn);
+    e.build_device(e.P_d.p, e.J_d.p, e.K_d.p, 1.0, e.stream);
+    geam(1.0, F.p, 1.0, e.J_d.p, false, T2.p);
+    geam(1.0, T2.p, -0.5, e.K_d.p, false, F.p);
+    const double E = enuc + 0.5 * dot(P.p, H.p) + 0.5 * dot(P.p, F.p);
+    // err = (F P S) - (F P S)^T, column-major
+    ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, F.p, n, P.p, n, &zero,
+                     T2.p, n),
+       "dgemm");
+    ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, T2.p, n, S.p, n, &zero,
+                     Pd.p, n),
+       "dgemm");
+    geam(1.0, Pd.p, -1.0, Pd.p, true, err.p);
+    rocblas_int imax = 0;
+    ok(rocblas_idamax(blas, n * n, err.p, 1, &imax), "idamax");
+    double emax = 0.0;
+    HIP_CHECK(hipMemcpyAsync(&emax, err.p + (imax - 1), sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    geam(1.0, Pprev.p, -1.0, P.p, false, T2.p);
+    double dn = 0.0;
+    ok(rocblas_dnrm2(blas, n * n, T2.p, 1, &dn), "dnrm2");
+    copy(P.p, Pprev.p, nn);
+    HIP_CHECK(hipStreamSynchronize(e.stream));
+    out[0] = E;
+    out[1] = std::fabs(emax) / n;
+    out[2] = dn / n;
+  }
+
+  // diagonalise X^T A X (A symmetric, n x n): C = X Cp, w = eigenvalues
+  void diag(const double* A) {
+    const double one = 1.0, zero = 0.0;
+    ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, n, &one, A, n, X.p, n, &zero, T1.p,
+                     n),
+       "dgemm");
+    ok(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, m, n, &one, X.p, n, T1.p, n, &zero,
+                     T2.p, m),
+       "dgemm");
+    const double half = 0.5;
+    ok(rocblas_dgeam(blas, rocblas_operation_none, rocblas_operation_transpose, m, m, &half, T2.p, m, &half, T2.p, m,
+                     Fo.p, m),
+       "dgeam");
+    ok(rocsolver_dsyevd(blas, rocblas_evect_original, rocblas_fill_upper, m, Fo.p, m, w.p, ework.p, info.p),
+       "dsyevd");
+    rocblas_int hinfo = 0;
+    HIP_CHECK(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, e.stream));
+    HIP_CHECK(hipStreamSynchronize(e.stream));
+    if (hinfo != 0) throw std::runtime_error("SCF: the divide-and-conquer eigensolve did not converge");
+    ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n, m, m, &one, X.p, n, Fo.p, m, &zero,
+                     C.p, n),
+       "dgemm");
+  }
+
+  void density() {
+    const double two = 2.0, zero = 0.0;
+    ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &two, C.p, n, C.p, n,
+                     &zero, P.p, n),
+       "dgemm");
+    HIP_CHECK(hipStreamSynchronize(e.stream));
+  }
+
+  // DIIS: store (F, err), extrapolate, diagonalise, new density
+  void update() {
+    const size_t nn = (size_t)n * n;
+    int slot;
+    if ((int)ring.size() < diis_size) {
+      slot = (int)ring.size();
+      ring.push_back(slot);
+    } else {
+      slot = ring.front();
+      ring.erase(ring.begin());
+      ring.push_back(slot);
+    }
+    Fh[slot].alloc(nn);
+    Eh[slot].alloc(nn);
+    copy(F.p, Fh[slot].p, nn);
+    copy(err.p, Eh[slot].p, nn);
+    for (int o : ring) {
+      const double v = dot(Eh[slot].p, Eh[o].p);
+      Bm[(size_t)slot * diis_size + o] = Bm[(size_t)o * diis_size + slot] = v;
+    }
+    const int k = (int)ring.size();
+    // [B -1; -1 0] c = [0; -1], solved by Gaussian elimination with partial
+    // pivoting (LU as in dgesv); an exactly singular system falls back to the
+    // newest Fock matrix
+    std::vector<double> A((size_t)(k + 1) * (k + 1), -1.0), rhs(k + 1, 0.0);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) A[(size_t)i * (k + 1) + j] = Bm[(size_t)ring[i] * diis_size + ring[j]];
+    A[(size_t)k * (k + 1) + k] = 0.0;
+    rhs[k] = -1.0;
+    std::vector<double> c(k, 0.0);
+    if (!solve_small(A, rhs, k + 1)) {
+      c[k - 1] = 1.0;
+    } else {
+      for (int i = 0; i < k; ++i) c[i] = rhs[i];
+    }
+    HIP_CHECK(hipMemsetAsync(T2.p, 0, nn * sizeof(double), e.stream));
+    for (int i = 0; i < k; ++i) ok(rocblas_daxpy(blas, n * n, &c[i], Fh[ring[i]].p, 1, T2.p, 1), "daxpy");
+    diag(T2.p);
+    density();
+  }
+
+  static bool solve_small(std::vector<double>& A, std::vector<double>& b, int N) {
+    for (int col = 0; col < N; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < N; ++r)
+        if (std::fabs(A[(size_t)r * N + col]) > std::fabs(A[(size_t)piv * N + col])) piv = r;
+      if (A[(size_t)piv * N + col] == 0.0) return false;  // exactly singular, as LAPACK dgesv reports
+      if (piv != col) {
+        for (int j = 0; j < N; ++j) std::swap(A[(size_t)piv * N + j], A[(size_t)col * N + j]);
+        std::swap(b[piv], b[col]);
+      }
+      for (int r = col + 1; r < N; ++r) {
+        const double f = A[(size_t)r * N + col] / A[(size_t)col * N + col];
+        for (int j = col; j < N; ++j) A[(size_t)r * N + j] -= f * A[(size_t)col * N + j];
+        b[r] -= f * b[col];
+      }
+    }
+    for (int r = N - 1; r >= 0; --r) {
+      double v = b[r];
+      for (int j = r + 1; j < N; ++j) v -= A[(size_t)r * N + j] * b[j];
+      b[r] = v / A[(size_t)r * N + r];
+    }
+    return true;
+  }
+
+  // final full rebuild and plain diagonalisation (scf_impl.cpp:612-621)
+  double finish() {
+    double out[3];
+    step(true, out);
+    diag(F.p);
+    return out[0];
+  }
+
+  // 0: P, 1: F (n x n); 2: C (n x norb, row-major); 3: eigenvalues (norb)
+  void get(int what, double* dst) {
+    const size_t nn = (size_t)n * n;
+    if (what == 0 || what == 1) {
+      HIP_CHECK(hipMemcpyAsync(dst, what ? F.p : P.p, nn * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+      HIP_CHECK(hipStreamSynchronize(e.stream));
+    } else if (what == 2) {
+      std::vector<double> cm((size_t)n * m);
+      HIP_CHECK(hipMemcpyAsync(cm.data(), C.p, cm.size() * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+      HIP_CHECK(hipStreamSynchronize(e.stream));
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) dst[(size_t)i * m + j] = cm[(size_t)j * n + i];
+    } else if (what == 3) {
+      HIP_CHECK(hipMemcpyAsync(dst, w.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+      HIP_CHECK(hipStreamSynchronize(e.stream));
+    } else {
+      throw std::invalid_argument("SCF get: what must be 0 (P), 1 (F), 2 (C) or 3 (eigenvalues)");
+    }
+  }
+};
 }  // namespace qfe
 
 struct qfe_dfj {
   std::unique_ptr<qfe::DFJ> e;
+};
+
+struct qfe_scf {
+  std::unique_ptr<qfe::SCFDevice> s;
 };
 
 extern "C" {
@@ -1470,6 +1722,59 @@ int qfe_mo_eri(qfe_handle* h, unsigned long long nb, unsigned long long nt, cons
     qfe::mo_transform(h->e.basis, h->e.device, (size_t)nt, Cs, out);
   });
 }
+
+int qfe_scf_create(qfe_handle* h, const double* H, const double* S, const double* X, int norb, int nocc,
+                   double enuc, int diis_size, qfe_scf** out) {
+  return guarded([&] {
+    if (!h || !H || !S || !X || !out) throw std::invalid_argument("scf_create: null argument");
+    *out = nullptr;
+    auto s = std::make_unique<qfe_scf>();
+    s->s = std::make_unique<qfe::SCFDevice>(h->e, H, S, X, norb, nocc, enuc, diis_size);
+    *out = s.release();
+  });
+}
+
+int qfe_scf_set_density(qfe_scf* s, const double* P) {
+  return guarded([&] {
+    if (!s || !P) throw std::invalid_argument("scf_set_density: null argument");
+    s->s->set_density(P);
+  });
+}
+
+int qfe_scf_step(qfe_scf* s, int full_build, double* energy, double* orbital_gradient, double* density_rms) {
+  return guarded([&] {
+    if (!s || !energy || !orbital_gradient || !density_rms) throw std::invalid_argument("scf_step: null argument");
+    double o[3];
+    s->s->step(full_build != 0, o);
+    if (!std::isfinite(o[0])) throw std::runtime_error("NaN or INF Encountered in SCF Energy");
+    *energy = o[0];
+    *orbital_gradient = o[1];
+    *density_rms = o[2];
+  });
+}
+
+int qfe_scf_update(qfe_scf* s) {
+  return guarded([&] {
+    if (!s) throw std::invalid_argument("scf_update: null handle");
+    s->s->update();
+  });
+}
+
+int qfe_scf_finish(qfe_scf* s, double* energy) {
+  return guarded([&] {
+    if (!s || !energy) throw std::invalid_argument("scf_finish: null argument");
+    *energy = s->s->finish();
+  });
+}
+
+int qfe_scf_get(qfe_scf* s, int what, double* out) {
+  return guarded([&] {
+    if (!s || !out) throw std::invalid_argument("scf_get: null argument");
+    s->s->get(what, out);
+  });
+}
+
+void qfe_scf_destroy(qfe_scf* s) { delete s; }
 
 void qfe_dfj_destroy(qfe_dfj* h) { delete h; }
 

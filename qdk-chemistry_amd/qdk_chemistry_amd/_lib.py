@@ -41,7 +41,8 @@ class QfeStats(ctypes.Structure):
 EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients", "qfe_one_body",
            "qfe_one_body_grad", "qfe_one_body_gpu", "qfe_one_body_grad_gpu", "qfe_eigh_gpu",
            "qfe_dfj_create", "qfe_dfj_build_j", "qfe_dfj_naux", "qfe_dfj_destroy",
-           "qfe_quarter_trans", "qfe_mo_eri", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_max_l", "qfe_destroy",
+           "qfe_quarter_trans", "qfe_mo_eri", "qfe_scf_create", "qfe_scf_set_density",
+           "qfe_scf_step", "qfe_scf_update", "qfe_scf_finish", "qfe_scf_get", "qfe_scf_destroy", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_max_l", "qfe_destroy",
            "qfe_last_error"]
 
 _lib = None
@@ -81,6 +82,15 @@ def lib():
         L.qfe_dfj_destroy.restype = None
         L.qfe_quarter_trans.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, _dp, _dp]
         L.qfe_mo_eri.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_ulonglong, _dp, _dp, _dp, _dp, _dp]
+        L.qfe_scf_create.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.qfe_scf_set_density.argtypes = [ctypes.c_void_p, _dp]
+        L.qfe_scf_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+        L.qfe_scf_update.argtypes = [ctypes.c_void_p]
+        L.qfe_scf_finish.argtypes = [ctypes.c_void_p, _dp]
+        L.qfe_scf_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.qfe_scf_destroy.argtypes = [ctypes.c_void_p]
+        L.qfe_scf_destroy.restype = None
         L.qfe_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(QfeStats)]
         L.qfe_set_precision.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.qfe_device_buffers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),

@@ -11,8 +11,9 @@ Restates the Fock-build step of SCFImpl::iterate_
     density_threshold = 1e2 * convergence_threshold and og_threshold =
     convergence_threshold (ms/scf.cpp:167-174)
   * orthogonalisation X = U s^-1/2 dropping s < max(s)/1e9 (:422-493)
-The orbital update is DIIS (scf_algorithm/diis.cpp) in FP64 on the host; the
-reference's default switches to GDM near convergence, which changes the
+The orbital update is DIIS (scf_algorithm/diis.cpp) in FP64, on the device
+by default (qfe_scf_*: F, P, the DIIS history, X^T F X, the eigensolve and
+the new density stay in HBM) or on the host; the reference's default switches to GDM near convergence, which changes the
 iteration path but not the converged energy.  After convergence a full
 (non-incremental) rebuild gives the final Fock matrix (:612-621).
 """
@@ -23,7 +24,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .eri import ERIConfig, create, eigh_gpu, one_body, one_body_gpu, one_body_grad, one_body_grad_gpu
+from . import _lib
+from .eri import ERIConfig, RysERI, create, eigh_gpu, one_body, one_body_gpu, one_body_grad, one_body_grad_gpu
 
 
 @dataclass
@@ -44,6 +46,9 @@ class SCFConfig:
     one_body_on_gpu: bool = True
     # diagonalise the orthogonalised Fock matrix on that GPU (rocSOLVER dsyevd)
     eigen_on_gpu: bool = True
+    # keep F, P, the DIIS history and the orbital update in the engine's HBM
+    # (qfe_scf_*) when the ERI engine is the GPU Rys engine
+    device_resident: bool = True
 
 
 def _one_body(basis, structure, op, cfg):
@@ -180,6 +185,8 @@ class RHF:
             P = self._density(C)
         else:
             P = np.array(P0, dtype=np.float64)
+        if cfg.device_resident and isinstance(self.eri, RysERI):
+            return self._run_device(P, verbose)
         conv = cfg.convergence_threshold
         dens_thr, og_thr = conv * 1e2, conv
         F = self.H.copy()
@@ -255,6 +262,75 @@ class RHF:
         E = self.energy(P, F)
         e, C = self._diag(F)
         self.result = dict(energy=E, P=P, F=F, C=C, eps=e, iterations=step + 1, converged=converged,
+                           build_seconds=float(np.sum(self.build_times)), mixed_builds=self.mixed_builds)
+        return self.result
+
+    def _run_device(self, P, verbose=False):
+        """The loop of run() with F, P and DIIS in the engine's HBM: the same
+        incremental-build schedule, convergence test, mixed-precision switch
+        and final full rebuild; only E, the orbital gradient and the density
+        change come back per iteration."""
+        import ctypes
+        cfg = self.cfg
+        L = _lib.lib()
+        X = np.ascontiguousarray(self.X, dtype=np.float64)
+        H = np.ascontiguousarray(self.H, dtype=np.float64)
+        S = np.ascontiguousarray(self.S, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n, norb = X.shape
+        h = ctypes.c_void_p()
+        _lib.check(L.qfe_scf_create(self.eri.h, _lib.ptr(H), _lib.ptr(S), _lib.ptr(X), norb, self.nocc,
+                                    self.E_nuc, cfg.diis_size, ctypes.byref(h)))
+        try:
+            _lib.check(L.qfe_scf_set_density(h, _lib.ptr(P)))
+            conv = cfg.convergence_threshold
+            dens_thr, og_thr = conv * 1e2, conv
+            mixed = bool(getattr(self.eri, "mixed", False))
+            self.mixed_builds = 0
+            reset_now = False
+            og = np.inf
+            E_last = 0.0
+            converged = False
+            cE, cog, cdr = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            for step in range(cfg.max_iterations):
+                if mixed and og < cfg.fp64_switch_gradient:
+                    self.eri.set_precision(False)
+                    mixed = False
+                    reset_now = True
+                full = step < cfg.incremental_fock_start_step or step % cfg.fock_reset_steps == 0 or reset_now
+                reset_now = False
+                if mixed:
+                    self.mixed_builds += 1
+                _lib.check(L.qfe_scf_step(h, int(full), ctypes.byref(cE), ctypes.byref(cog), ctypes.byref(cdr)))
+                E, og, drms = cE.value, cog.value, cdr.value
+                st = self.eri.stats()
+                self.build_times.append(st["ms_total"] / 1e3)
+                self.build_stats.append(st)
+                if verbose:
+                    print(f"Step {step:03d}: E={E:.15e}, DE={E - E_last:+.3e}, |DP|={drms:.3e}, |DG|={og:.3e}")
+                E_last = E
+                if drms < dens_thr and og < og_thr:
+                    converged = True
+                    break
+                _lib.check(L.qfe_scf_update(h))
+            if not converged:
+                raise RuntimeError(f"SCF failed to converge after {cfg.max_iterations} steps")
+            if mixed:
+                self.eri.set_precision(False)
+            e_out = ctypes.c_double()
+            _lib.check(L.qfe_scf_finish(h, ctypes.byref(e_out)))
+            st = self.eri.stats()
+            self.build_times.append(st["ms_total"] / 1e3)
+            self.build_stats.append(st)
+            P = np.empty((n, n))
+            F = np.empty((n, n))
+            C = np.empty((n, norb))
+            e = np.empty(norb)
+            for what, arr in ((0, P), (1, F), (2, C), (3, e)):
+                _lib.check(L.qfe_scf_get(h, what, _lib.ptr(arr)))
+        finally:
+            L.qfe_scf_destroy(h)
+        self.result = dict(energy=e_out.value, P=P, F=F, C=C, eps=e, iterations=step + 1, converged=converged,
                            build_seconds=float(np.sum(self.build_times)), mixed_builds=self.mixed_builds)
         return self.result
 

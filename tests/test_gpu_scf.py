@@ -112,3 +112,46 @@ def test_mixed_precision_scf_energy():
     print("E64 %.10f Emix %.10f dE %.2e mixed builds %d" % (e64, r["energy"], r["energy"] - e64, r["mixed_builds"]))
     assert r["mixed_builds"] > 0
     assert abs(r["energy"] - e64) <= 1e-6
+
+
+@pytest.mark.parametrize("mix", [False, True])
+def test_device_resident_scf_matches_host_loop(mix):
+    """The device-resident iterations (qfe_scf_*) follow the host loop: same
+    iteration count, energy within 1e-9, converged F within 1e-8 and orbital
+    energies within 1e-8, on (H2O)4/cc-pVDZ, FP64 and mixed precision."""
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig
+    from qdk_chemistry_amd.molecule import water_cluster
+    from qdk_chemistry_amd.scf import RHF, SCFConfig
+    mol = water_cluster(4, 11)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    out = []
+    for dev in (True, False):
+        cfg = SCFConfig(convergence_threshold=1e-9, device_resident=dev, eri=ERIConfig(is_mix_precision=mix))
+        out.append(RHF(mol, b, cfg).run())
+    d, h = out
+    print("device %.12f (%d it) host %.12f (%d it)" % (d["energy"], d["iterations"], h["energy"], h["iterations"]),
+          flush=True)
+    assert abs(d["energy"] - h["energy"]) < 1e-9
+    assert abs(d["iterations"] - h["iterations"]) <= 1
+    assert np.max(np.abs(d["F"] - h["F"])) < 1e-8
+    assert np.max(np.abs(d["eps"] - h["eps"])) < 1e-8
+    nocc = mol.nelectrons // 2
+    Pd = 2.0 * d["C"][:, :nocc] @ d["C"][:, :nocc].T
+    assert np.max(np.abs(Pd - d["P"])) < 1e-6
+
+
+def test_device_resident_scf_rejects_bad_shapes(water):
+    import ctypes
+    from qdk_chemistry_amd import _lib
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import create
+    b = load_basis("cc-pvdz", water.numbers, water.coords)
+    eri = create(b, water)
+    n = b.nbf
+    M = np.eye(n)
+    h = ctypes.c_void_p()
+    for norb, nocc in ((0, 5), (n + 1, 5), (n, 0), (n, n + 1)):
+        rc = _lib.lib().qfe_scf_create(eri.h, _lib.ptr(M), _lib.ptr(M), _lib.ptr(M), norb, nocc, 0.0, 10,
+                                       ctypes.byref(h))
+        assert rc == _lib.QFE_ERR_INVALID, (norb, nocc)
