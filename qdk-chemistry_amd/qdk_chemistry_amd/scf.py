@@ -125,8 +125,11 @@ def atom_guess(structure, basis_name, pure=True, eri_cfg=None, iterations=30):
     return P
 
 
-def orthogonalizer(S, lindep=-1.0):
-    s, U = np.linalg.eigh(S)
+def orthogonalizer(S, lindep=-1.0, device_id=None):
+    """X = U s^-1/2 over the eigenvalues s >= max(s) / 1e9 (or >= lindep)
+    of S (scf_impl.cpp:422-493); the eigensolve on the GPU when device_id
+    is given."""
+    s, U = np.linalg.eigh(S) if device_id is None else eigh_gpu(S, device_id)
     thr = lindep if lindep >= 0 else s.max() / 1e9
     keep = s >= thr
     return U[:, keep] / np.sqrt(s[keep])
@@ -145,7 +148,8 @@ class RHF:
             raise ValueError("RHF needs an even number of electrons")
         self.S = _one_body(basis, structure, 0, self.cfg)
         self.H = _one_body(basis, structure, 1, self.cfg) + _one_body(basis, structure, 2, self.cfg)
-        self.X = orthogonalizer(self.S, self.cfg.lindep_threshold)
+        self.X = orthogonalizer(self.S, self.cfg.lindep_threshold,
+                                self.cfg.eri.device_id if self.cfg.eigen_on_gpu else None)
         self.E_nuc = structure.nuclear_repulsion()
         self.build_times = []
         self.build_stats = []
