@@ -216,6 +216,40 @@ def mo_transform_timings(nwat, nt, host=True):
     return out
 
 
+def dfj_timings(nwat):
+    """Seconds for density-fitted J on a water cluster in def2-SVP with
+    def2-universal-jfit: the set-up (three-centre integrals, metric and its
+    Cholesky factor, once) and one J build, beside one exact J/K build of the
+    Rys engine at the same density; and the fitting error of J."""
+    import numpy as np
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import DFJ, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(nwat, 5000 + nwat)
+    b = load_basis("def2-svp", mol.numbers, mol.coords)
+    aux = load_basis("def2-universal-jfit", mol.numbers, mol.coords)
+    rng = np.random.Generator(np.random.PCG64(nwat))
+    A = rng.uniform(-0.05, 0.05, (b.nbf, b.nbf))
+    P = A + A.T
+    t = time.perf_counter()
+    df = DFJ(b, aux, mol)
+    t_setup = time.perf_counter() - t
+    df.build_JK(P)
+    t = time.perf_counter()
+    Jd, _ = df.build_JK(P)
+    t_j = time.perf_counter() - t
+    eri = create(b, mol)
+    eri.build_JK(P)
+    t = time.perf_counter()
+    Je, _ = eri.build_JK(P)
+    t_jk = time.perf_counter() - t
+    out = dict(workload=f"(H2O){nwat} def2-SVP / def2-universal-jfit, nbf {b.nbf}, naux {df.naux}",
+               setup_s=t_setup, build_j_s=t_j, exact_jk_build_s=t_jk,
+               max_rel_j_error=float(np.max(np.abs(Jd - Je)) / np.max(np.abs(Je))))
+    log("DF-J: " + ", ".join(f"{k} {v:.3g}" if isinstance(v, float) else f"{k} {v}" for k, v in out.items()))
+    return out
+
+
 def cpu_baseline(basis, mol, P, stride):
     from oracle.oracle import OracleERI
     threads = min(16, os.cpu_count() or 1)
@@ -243,6 +277,8 @@ def main():
     ap.add_argument("--mo-waters", type=int, default=4, help="water cluster of the MO-transform timing (0: skip)")
     ap.add_argument("--mo-nt", type=int, default=48)
     ap.add_argument("--mo-only", action="store_true", help="only the MO-transform timing, one JSON line")
+    ap.add_argument("--dfj-waters", type=int, default=16, help="water cluster of the DF-J timing (0: skip)")
+    ap.add_argument("--dfj-only", action="store_true", help="only the DF-J timing, one JSON line")
     ap.add_argument("--density", help="load the density from this .npy instead of running the SCF")
     ap.add_argument("--save-density", help="write the converged density to this .npy")
     args = ap.parse_args()
@@ -250,6 +286,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-device "
               f"in this build.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.dfj_only:
+        print(json.dumps({"dfj": dfj_timings(args.dfj_waters)}), flush=True)
+        return
     if args.mo_only:
         print(json.dumps({"mo_transform": mo_transform_timings(args.mo_waters, args.mo_nt, not args.no_cpu)}),
               flush=True)
@@ -361,6 +400,8 @@ def main():
     out["config"].update(result)
     if "scf_fp64" in result:
         out["config"]["harness"] = harness_timings(basis, mol, P, not args.no_cpu)
+    if args.dfj_waters > 0:
+        out["config"]["dfj"] = dfj_timings(args.dfj_waters)
     if args.mo_waters > 0:
         out["config"]["mo_transform"] = mo_transform_timings(args.mo_waters, args.mo_nt, not args.no_cpu)
     if not args.no_cpu:

@@ -259,6 +259,24 @@ __device__ void ob_contract(const OneBodyView& b, int s1, int s2, const double* 
   }
 }
 
+// true when every primitive pair of (s1, s2) has |c_a c_b| exp(-ab/p |AB|^2)
+// below thr: such a pair's S, T, V and derivative blocks are below
+// thr times factors of order (pi/p)^(3/2), Z and the angular polynomials
+__device__ bool ob_pair_negligible(const OneBodyView& b, int s1, int s2, double thr) {
+  const double* A = b.cen + 3 * s1;
+  const double* B = b.cen + 3 * s2;
+  double AB2 = 0.0;
+  for (int d = 0; d < 3; ++d) AB2 += (A[d] - B[d]) * (A[d] - B[d]);
+  for (int ia = b.po[s1]; ia < b.po[s1 + 1]; ++ia)
+    for (int ib = b.po[s2]; ib < b.po[s2 + 1]; ++ib) {
+      const double a = b.exps[ia], bb = b.exps[ib];
+      if (fabs(b.coefs[ia] * b.coefs[ib]) * exp(-a * bb / (a + bb) * AB2) >= thr) return false;
+    }
+  return true;
+}
+
+constexpr double kPairThr = 1e-17;
+
 }  // namespace
 
 // one thread per shell pair s1 >= s2: the block and its transpose
@@ -301,25 +319,47 @@ __global__ __launch_bounds__(256) void k_one_body_grad_st(OneBodyView b, RysArgs
 
 // V derivative: one thread per (ordered shell pair, nucleus); by translation
 // invariance the nucleus takes minus the centre derivatives of both shells,
-// and the pair (s2, s1) supplies the second one
+// and the pair (s2, s1) supplies the second one.  Pairs whose every
+// primitive product prefactor is below 1e-17 are skipped (most pairs of a
+// large molecule: their contributions are below 1e-14).
 __global__ __launch_bounds__(256) void k_one_body_grad_v(OneBodyView b, RysArgs ra, const double* __restrict__ W,
                                                          double* __restrict__ grad) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long np = (long long)b.nsh * b.nsh;
-  if (t >= np * b.natoms) return;
-  const int cn = (int)(t / np);
-  const long long pq = t % np;
-  const int s1 = (int)(pq / b.nsh), s2 = (int)(pq % b.nsh);
-  double d[3 * kMaxC * kMaxF];
-  ob_deriv_A(b, ra, s1, s2, 2, cn, d);
-  double g[3];
-  ob_contract(b, s1, s2, d, W, g);
-  const int at = b.atom[s1];
-  for (int x = 0; x < 3; ++x)
-    if (g[x] != 0.0) {
-      atomicAdd(&grad[3 * at + x], g[x]);
-      atomicAdd(&grad[3 * cn + x], -g[x]);
+  const bool valid = t < np * b.natoms;
+  // the nucleus is the fast index: a wave works on one shell pair, so its
+  // control flow (angular momenta, contraction lengths, the skip) is uniform
+  // and its contributions to the pair's own atom are summed across the wave
+  // before one atomic per component
+  int cn = 0, s1 = 0;
+  long long pq = -1;
+  double g[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    cn = (int)(t % b.natoms);
+    pq = t / b.natoms;
+    s1 = (int)(pq / b.nsh);
+    const int s2 = (int)(pq % b.nsh);
+    if (!ob_pair_negligible(b, s1, s2, kPairThr)) {
+      double d[3 * kMaxC * kMaxF];
+      ob_deriv_A(b, ra, s1, s2, 2, cn, d);
+      ob_contract(b, s1, s2, d, W, g);
+      for (int x = 0; x < 3; ++x)
+        if (g[x] != 0.0) atomicAdd(&grad[3 * cn + x], -g[x]);
     }
+  }
+  const long long pq0 = __shfl(pq, 0);
+  if (__all(pq == pq0)) {
+    if (pq0 < 0) return;  // the whole wave is past the end
+    for (int x = 0; x < 3; ++x) {
+      double v = g[x];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (threadIdx.x % 64 == 0 && v != 0.0) atomicAdd(&grad[3 * b.atom[s1] + x], v);
+    }
+  } else if (valid) {
+    const int at = b.atom[s1];
+    for (int x = 0; x < 3; ++x)
+      if (g[x] != 0.0) atomicAdd(&grad[3 * at + x], g[x]);
+  }
 }
 
 void launch_one_body(const OneBodyView& b, const RysArgs& ra, int op, double* out, hipStream_t s) {
