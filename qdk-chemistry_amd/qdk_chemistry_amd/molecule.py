@@ -23,6 +23,7 @@ ANGSTROM_TO_BOHR = 1.0 / BOHR_TO_ANGSTROM
 class Structure:
     numbers: np.ndarray       # nuclear charges (int)
     coords: np.ndarray        # natoms x 3, bohr
+    charge: int = 0           # molecular charge
 
     @property
     def natoms(self) -> int:
@@ -30,7 +31,7 @@ class Structure:
 
     @property
     def nelectrons(self) -> int:
-        return int(np.sum(self.numbers))
+        return int(np.sum(self.numbers)) - int(self.charge)
 
     def nuclear_repulsion(self) -> float:
         e = 0.0

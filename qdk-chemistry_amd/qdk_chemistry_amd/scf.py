@@ -358,3 +358,169 @@ class RHF:
         na = m.natoms
         self.grad_parts = dict(dJ=dJ.reshape(3, na).T.copy(), dK=dK.reshape(3, na).T.copy())
         return g + (dJ + dK).reshape(3, na).T
+
+
+class UHF:
+    """Open-shell SCF through the engine's two-density build (ndm = 2,
+    ERI::build_JK with has_spin_split_density, eri.h:79-80): J and K of the
+    alpha and beta densities in one call, F_s = H + J(P_a) + J(P_b) - K(P_s),
+    E = E_nuc + 1/2 sum_s Tr(P_s (H + F_s)).  The orbital update is DIIS over
+    both spins (errors F_s P_s S - S P_s F_s, one set of coefficients); the
+    guess is given densities, the atomic density split evenly, the core
+    Hamiltonian, or ("rohf") the converged ROHF densities of the same
+    occupation, which steer the iterations to the ground state where the
+    other guesses can settle in an excited one.
+    Convergence: |dE| below the threshold squared and max|error| below
+    the threshold.  n_alpha = (N + M - 1) / 2 and n_beta = (N - M + 1) / 2
+    for N electrons and multiplicity M."""
+
+    def __init__(self, structure, basis, multiplicity, cfg: SCFConfig | None = None, eri=None):
+        if np.any(getattr(basis, "ecp_electrons", 0)):
+            raise NotImplementedError("effective core potentials: the ECP integrals are not part of this package")
+        self.structure = structure
+        self.basis = basis
+        self.cfg = cfg or SCFConfig()
+        self.eri = eri or create(basis, structure, self.cfg.eri)
+        ne = structure.nelectrons
+        if multiplicity < 1 or (ne + multiplicity - 1) % 2 or multiplicity - 1 > ne:
+            raise ValueError(f"multiplicity {multiplicity} does not fit {ne} electrons")
+        self.na = (ne + multiplicity - 1) // 2
+        self.nb = (ne - multiplicity + 1) // 2
+        self.S = _one_body(basis, structure, 0, self.cfg)
+        self.H = _one_body(basis, structure, 1, self.cfg) + _one_body(basis, structure, 2, self.cfg)
+        self.X = orthogonalizer(self.S, self.cfg.lindep_threshold,
+                                self.cfg.eri.device_id if self.cfg.eigen_on_gpu else None)
+        self.E_nuc = structure.nuclear_repulsion()
+
+    def _eig(self, F):
+        Fo = self.X.T @ F @ self.X
+        Fo = 0.5 * (Fo + Fo.T)
+        e, Cp = eigh_gpu(Fo, self.cfg.eri.device_id) if self.cfg.eigen_on_gpu else np.linalg.eigh(Fo)
+        return e, self.X @ Cp
+
+    def _fock(self, Pa, Pb):
+        J, K = self.eri.build_JK(np.stack([Pa, Pb]), 1.0, 0.0, 0.0)
+        Jt = J[0] + J[1]
+        Fa, Fb = self.H + Jt - K[0], self.H + Jt - K[1]
+        E = self.E_nuc + 0.5 * float(np.sum(Pa * (self.H + Fa)) + np.sum(Pb * (self.H + Fb)))
+        return Fa, Fb, E
+
+    def _guess(self, guess):
+        if isinstance(guess, tuple):  # (P_alpha, P_beta)
+            return np.array(guess[0], dtype=np.float64), np.array(guess[1], dtype=np.float64)
+        if guess == "atom":
+            P = atom_guess(self.structure, self.basis.name, self.basis.pure, self.cfg.eri)
+            return 0.5 * P, 0.5 * P
+        _, C = self._eig(self.H)
+        return C[:, :self.na] @ C[:, :self.na].T, C[:, :self.nb] @ C[:, :self.nb].T
+
+    def _update(self, Fa, Fb, Pa, Pb, diis, errs):
+        """DIIS-extrapolated Fock matrices -> new densities (and orbitals)."""
+        Fa_d, Fb_d = self._diis(diis, errs, [Fa, Fb])
+        ea, Ca = self._eig(Fa_d)
+        eb, Cb = self._eig(Fb_d)
+        return Ca[:, :self.na] @ Ca[:, :self.na].T, Cb[:, :self.nb] @ Cb[:, :self.nb].T, (ea, Ca, eb, Cb)
+
+    def _errors(self, Fa, Fb, Pa, Pb):
+        ea = Fa @ Pa @ self.S
+        eb = Fb @ Pb @ self.S
+        return [ea - ea.T, eb - eb.T]
+
+    def _diis(self, hist, errs, Fs):
+        hist.append((Fs, errs))
+        if len(hist) > self.cfg.diis_size:
+            hist.pop(0)
+        m = len(hist)
+        B = -np.ones((m + 1, m + 1))
+        B[m, m] = 0.0
+        for i in range(m):
+            for j in range(i + 1):
+                B[i, j] = B[j, i] = sum(float(np.sum(a * b)) for a, b in zip(hist[i][1], hist[j][1]))
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1.0
+        try:
+            c = np.linalg.solve(B, rhs)[:m]
+        except np.linalg.LinAlgError:
+            c = np.zeros(m)
+            c[-1] = 1.0
+        return [sum(ci * h[0][k] for ci, h in zip(c, hist)) for k in range(len(Fs))]
+
+    def run(self, guess="atom", verbose=False):
+        cfg = self.cfg
+        conv = cfg.convergence_threshold
+        if isinstance(guess, str) and guess.startswith("rohf"):
+            inner = guess.split(":", 1)[1] if ":" in guess else "atom"
+            r = ROHF(self.structure, self.basis, self.na - self.nb + 1, cfg, eri=self.eri).run(guess=inner)
+            guess = (r["Pa"], r["Pb"])
+        Pa, Pb = self._guess(guess)
+        E_last = 0.0
+        hist = []
+        converged = False
+        for step in range(cfg.max_iterations):
+            Fa, Fb, E = self._fock(Pa, Pb)
+            if not np.isfinite(E):
+                raise RuntimeError("NaN or INF Encountered in SCF Energy")
+            errs = self._errors(Fa, Fb, Pa, Pb)
+            gmax = max(float(np.max(np.abs(e))) for e in errs)
+            if verbose:
+                print(f"Step {step:03d}: E={E:.15e}, DE={E - E_last:+.3e}, |err|={gmax:.3e}")
+            if step > 0 and abs(E - E_last) < conv * conv and gmax < conv:
+                converged = True
+                break
+            E_last = E
+            Pa, Pb, orbs = self._update(Fa, Fb, Pa, Pb, hist, errs)
+        if not converged:
+            raise RuntimeError(f"SCF failed to converge after {cfg.max_iterations} steps")
+        self.result = dict(energy=E, Pa=Pa, Pb=Pb, Fa=Fa, Fb=Fb, iterations=step + 1, converged=converged)
+        return self.result
+
+
+class ROHF(UHF):
+    """Restricted open-shell HF on the same two-density build: one set of
+    orbitals, n_beta doubly and n_alpha - n_beta singly occupied.  The
+    orbital update diagonalises the effective Fock matrix whose blocks in the
+    current orbitals are (F_a + F_b)/2 within the closed, open and virtual
+    spaces and between closed and virtual, F_b between closed and open, and
+    F_a between open and virtual; DIIS runs on that matrix with the error
+    F_eff P S - S P F_eff, P = P_a + P_b (it vanishes exactly when those
+    coupling blocks do).  The converged energy is that of the ROHF stationary
+    state, independent of the choice of the diagonal blocks."""
+
+    def _errors(self, Fa, Fb, Pa, Pb):
+        Feff = self._effective(Fa, Fb)
+        e = Feff @ (Pa + Pb) @ self.S
+        self._Feff = Feff
+        return [e - e.T]
+
+    def _effective(self, Fa, Fb):
+        C = self._C
+        na, nb = self.na, self.nb
+        Fc = 0.5 * (Fa + Fb)
+        Fm = C.T @ Fc @ C
+        Fbm = C.T @ Fb @ C
+        Fam = C.T @ Fa @ C
+        R = Fm.copy()
+        R[:nb, nb:na] = Fbm[:nb, nb:na]
+        R[nb:na, :nb] = Fbm[nb:na, :nb]
+        R[nb:na, na:] = Fam[nb:na, na:]
+        R[na:, nb:na] = Fam[na:, nb:na]
+        SC = self.S @ C
+        return SC @ R @ SC.T
+
+    def _guess(self, guess):
+        # one set of orbitals: those of the core Hamiltonian, or of the
+        # closed-shell Fock matrix of the atomic density
+        F = self.H
+        if guess == "atom":
+            Pa, Pb = super()._guess(guess)
+            J, K = self.eri.build_JK(Pa + Pb, 1.0, 0.0, 0.0)
+            F = self.H + J - 0.5 * K
+        _, self._C = self._eig(F)
+        C = self._C
+        return C[:, :self.na] @ C[:, :self.na].T, C[:, :self.nb] @ C[:, :self.nb].T
+
+    def _update(self, Fa, Fb, Pa, Pb, diis, errs):
+        (Fd,) = self._diis(diis, errs, [self._Feff])
+        e, C = self._eig(Fd)
+        self._C = C
+        return C[:, :self.na] @ C[:, :self.na].T, C[:, :self.nb] @ C[:, :self.nb].T, (e, C)

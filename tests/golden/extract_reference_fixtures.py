@@ -5,6 +5,8 @@ Sources (data only): scf/tests/data/h2o.json and water10.json (RHF/HF/6-31g*/
 pure energy + gradient, atom-major), geometries from scf/tests/test_common.h
 (make_h2o, make_water10, make_TeH2_HI; angstrom with the local bohr factor
 0.52917721092); scf/tests/data/h2o_gauxc.json (RHF with density-fitted J);
+cpp/tests/test_scf.cpp (charged and open-shell RHF/UHF/ROHF energies of
+Li, O2, OH and the O atom, structures from cpp/tests/ut_common.hpp);
 cpp/tests/test_mp2.cpp (RMP2 correlation energy of singlet O2 in
 cc-pVDZ at the test's default bond length, a PySCF value, with the
 tolerance testing::mp2_tolerance of cpp/tests/ut_common.hpp);
@@ -60,6 +62,46 @@ def main():
                           open(os.path.join(TESTS, "ut_common.hpp")).read()).group(1))
     out["o2_rmp2"] = dict(numbers=[8, 8], coords_bohr=[[0.0, 0.0, 0.0], [bond, 0.0, 0.0]], basis="cc-pvdz",
                           pure=True, correlation_energy=corr, tolerance=tol)
+    # open-shell and charged SCF energies of cpp/tests/test_scf.cpp with the
+    # structures of cpp/tests/ut_common.hpp (angstrom; CODATA-2022 bohr)
+    scf = open(os.path.join(TESTS, "test_scf.cpp")).read()
+    ut = open(os.path.join(TESTS, "ut_common.hpp")).read()
+
+    def test_body(name):
+        b = scf[scf.index(f"TEST_F(ScfTest, {name})"):]
+        return b[: b.index("\n}\n")]
+
+    def structure(fn):
+        b = ut[ut.index(f"inline std::shared_ptr<Structure> {fn}()"):]
+        b = b[: b.index("\n}\n")]
+        xyz = [[float(v) for v in t] for t in re.findall(r"\{\s*([-\d.]+),\s*([-\d.]+),\s*([-\d.]+)\s*\}", b)]
+        els = re.findall(r"Element::(\w+)", b)
+        return els, xyz
+
+    def near(body, var):
+        return float(re.search(rf"EXPECT_NEAR\({var},\s*(-?[\d.]+)", body).group(1))
+
+    open_shell = []
+    li, oxy, oh, o2 = (structure(f) for f in ("create_li_structure", "create_oxygen_structure",
+                                              "create_oh_structure", "create_o2_structure"))
+    b = test_body("Lithium")
+    open_shell.append(dict(name="Li doublet", test="Lithium", structure=li, charge=0, multiplicity=2,
+                           basis="def2-svp", kind="UHF", energy=near(b, "E_default")))
+    open_shell.append(dict(name="Li+ singlet", test="Lithium", structure=li, charge=1, multiplicity=1,
+                           basis="def2-svp", kind="RHF", energy=near(b, "E_li_plus_1")))
+    b = test_body("Oxygen")
+    open_shell.append(dict(name="O2 singlet", test="Oxygen", structure=o2, charge=0, multiplicity=1,
+                           basis="sto-3g", kind="RHF", energy=near(b, "E_singlet")))
+    open_shell.append(dict(name="O2 triplet", test="Oxygen", structure=o2, charge=0, multiplicity=3,
+                           basis="sto-3g", kind="UHF", energy=near(b, "E_triplet")))
+    b = test_body("OH_ROHF_DIIS")
+    open_shell.append(dict(name="OH doublet", test="OH_ROHF_DIIS", structure=oh, charge=0, multiplicity=2,
+                           basis="sto-3g", kind="ROHF", energy=near(b, "E_doublet")))
+    b = test_body("Oxygen_atom_ROHF_GDM")
+    open_shell.append(dict(name="O triplet", test="Oxygen_atom_ROHF_GDM", structure=oxy, charge=0,
+                           multiplicity=3, basis="cc-pvdz", kind="ROHF", energy=near(b, "E_triplet")))
+    tol = float(re.search(r"scf_energy_tolerance = ([\d.e+-]+);", ut).group(1))
+    out["open_shell"] = dict(bohr_to_angstrom=0.529177210544, tolerance=tol, cases=open_shell)
     json.dump(out, open(os.path.join(HERE, "reference_fixtures.json"), "w"), indent=1)
     import numpy as np
     rec = json.load(open(os.path.join(ROOT, "data", "int1e.json")))
