@@ -158,7 +158,10 @@ static void init_boys(void) {
   }
 }
 
+static void oracle_init(void);
+
 void oracle_boys(int mmax, double T, double *F) {
+  oracle_init(); /* the table is built on first use, also without a handle */
   if (T < OR_BOYS_TMAX - 1.0) {
     int g = (int)(T / OR_BOYS_DT + 0.5);
     double d = g * OR_BOYS_DT - T; /* F_m(T) = sum_k F_{m+k}(T0) (T0-T)^k / k! */
