@@ -1,7 +1,9 @@
 // Density-fitted Coulomb (DF-J) on the GPU: the three-centre integrals
-// B[P][mu][nu] = (mu nu|P) over an auxiliary basis and the two-centre metric
-// V[P][Q] = (P|Q), kept in HBM (the in-core layout of the reference's
-// INCORE DF engine, scf/src/eri/INCORE/incore_impl_df.cpp:53-80).  The
+// B[P][mu nu] = (mu nu|P) over an auxiliary basis, mu >= nu only (the packed
+// lower triangle: half the HBM and half the bytes a build reads, against the
+// full square of the reference's INCORE DF engine,
+// scf/src/eri/INCORE/incore_impl_df.cpp:53-80), and the two-centre metric
+// V[P][Q] = (P|Q), kept in HBM.  The
 // build itself (X = B . vec(D), V c = X, J = B^T c; incore_impl_df.cpp:
 // 81-200) runs in engine.cpp with rocBLAS and rocSOLVER.
 //
@@ -200,7 +202,8 @@ __device__ ShellRef df_shell(const DFBasisView& v, int s) {
 
 }  // namespace
 
-// B[P][mu][nu] for one (orbital shell pair s1 >= s2, auxiliary shell) per thread
+// B[P][mu >= nu] (packed lower triangle, element mu (mu + 1) / 2 + nu) for one
+// (orbital shell pair s1 >= s2, auxiliary shell) per thread
 __global__ __launch_bounds__(256) void k_df_three_centre(DFBasisView o, DFBasisView x, RysArgs ra,
                                                          const double* __restrict__ sph, double* __restrict__ B) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,15 +220,14 @@ __global__ __launch_bounds__(256) void k_df_three_centre(DFBasisView o, DFBasisV
   df_cart(a, b, c, ra, cart);
   df_to_out(sph, a.l, b.l, c.l, o.pure, x.pure, cart, blk);
   const int ma = df_nfun(a.l, o.pure), mb = df_nfun(b.l, o.pure), mc = df_nfun(c.l, x.pure);
-  const size_t n = (size_t)o.nbf;
+  const size_t n = (size_t)o.nbf, np = n * (n + 1) / 2;
   for (int i = 0; i < ma; ++i)
-    for (int j = 0; j < mb; ++j)
-      for (int k = 0; k < mc; ++k) {
-        double* Bp = B + (size_t)(x.bf0[sc] + k) * n * n;
-        const double v = blk[(i * mb + j) * mc + k];
-        Bp[(size_t)(o.bf0[s1] + i) * n + o.bf0[s2] + j] = v;
-        Bp[(size_t)(o.bf0[s2] + j) * n + o.bf0[s1] + i] = v;
-      }
+    for (int j = 0; j < mb; ++j) {
+      const size_t mu = (size_t)o.bf0[s1] + i, nu = (size_t)o.bf0[s2] + j;
+      const size_t hi = mu > nu ? mu : nu, lo = mu > nu ? nu : mu;
+      const size_t e = hi * (hi + 1) / 2 + lo;
+      for (int k = 0; k < mc; ++k) B[(size_t)(x.bf0[sc] + k) * np + e] = blk[(i * mb + j) * mc + k];
+    }
 }
 
 // V[P][Q] for one auxiliary shell pair c1 >= c2 per thread: (c1 s0 | c2)

@@ -20,8 +20,9 @@ struct DFBasisView {
   int nsh, nbf, pure;
 };
 
-// B[P][mu][nu] = (mu nu | P), naux x nbf x nbf (orbital shells up to d,
-// auxiliary shells up to g); sph = the solid-harmonic table kSph (device)
+// B[P][mu (mu + 1) / 2 + nu] = (mu nu | P) for mu >= nu, naux x nbf (nbf + 1) / 2
+// (orbital shells up to d, auxiliary shells up to g); sph = the
+// solid-harmonic table kSph (device)
 void launch_df_three_centre(const DFBasisView& orb, const DFBasisView& aux, const RysArgs& ra, const double* sph,
                             double* B, hipStream_t s);
 // V[P][Q] = (P | Q), naux x naux

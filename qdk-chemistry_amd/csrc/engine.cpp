@@ -1017,9 +1017,10 @@ static rocblas_handle solver_handle(int device) {
 static void one_body_device(int device_id);
 
 // Density-fitted J (the reference's INCORE DF engine, incore_impl_df.cpp):
-// B[P][mu][nu] = (mu nu|P) and the Cholesky factor of V[P][Q] = (P|Q) kept in
-// HBM; a build is X = B . vec(D), V c = X, J = B^T c (two GEMVs and a
-// triangular solve pair).
+// B[P][mu >= nu] = (mu nu|P) (packed lower triangle) and the Cholesky factor
+// of V[P][Q] = (P|Q) kept in HBM; a build is X = B . d, V c = X, J = B^T c
+// (two GEMVs over the packed B and a triangular solve pair), with d the
+// density packed as D_mu,nu + D_nu,mu below the diagonal and D_mu,mu on it.
 struct DFJ {
   int device = 0, nbf = 0, naux = 0;
   DevBuf<double> B, V, d, x, j;
@@ -1054,19 +1055,27 @@ struct DFJ {
 
   void build_j(const double* P, double* J) {
     HIP_CHECK(hipSetDevice(device));
-    const size_t nn = (size_t)nbf * nbf;
-    HIP_CHECK(hipMemcpy(d.p, P, sizeof(double) * nn, hipMemcpyHostToDevice));
+    const size_t n = (size_t)nbf, np = n * (n + 1) / 2;
+    // D packed like B: D_mu,nu + D_nu,mu below the diagonal, D_mu,mu on it
+    std::vector<double> dp(np), jp(np);
+    for (size_t mu = 0; mu < n; ++mu) {
+      for (size_t nu = 0; nu < mu; ++nu) dp[mu * (mu + 1) / 2 + nu] = P[mu * n + nu] + P[nu * n + mu];
+      dp[mu * (mu + 1) / 2 + mu] = P[mu * n + mu];
+    }
+    HIP_CHECK(hipMemcpy(d.p, dp.data(), sizeof(double) * np, hipMemcpyHostToDevice));
     const double one = 1.0, zero = 0.0;
-    // B is row-major [naux][nbf^2], i.e. column-major nbf^2 x naux
-    if (rocblas_dgemv(blas, rocblas_operation_transpose, (rocblas_int)nn, naux, &one, B.p, (rocblas_int)nn, d.p, 1,
+    // B is row-major [naux][np], i.e. column-major np x naux
+    if (rocblas_dgemv(blas, rocblas_operation_transpose, (rocblas_int)np, naux, &one, B.p, (rocblas_int)np, d.p, 1,
                       &zero, x.p, 1) != rocblas_status_success)
       throw DeviceError("DF-J: rocblas_dgemv failed");
     if (rocsolver_dpotrs(blas, rocblas_fill_lower, naux, 1, V.p, naux, x.p, naux) != rocblas_status_success)
       throw DeviceError("DF-J: rocsolver_dpotrs failed");
-    if (rocblas_dgemv(blas, rocblas_operation_none, (rocblas_int)nn, naux, &one, B.p, (rocblas_int)nn, x.p, 1, &zero,
+    if (rocblas_dgemv(blas, rocblas_operation_none, (rocblas_int)np, naux, &one, B.p, (rocblas_int)np, x.p, 1, &zero,
                       j.p, 1) != rocblas_status_success)
       throw DeviceError("DF-J: rocblas_dgemv failed");
-    HIP_CHECK(hipMemcpy(J, j.p, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(jp.data(), j.p, sizeof(double) * np, hipMemcpyDeviceToHost));
+    for (size_t mu = 0; mu < n; ++mu)
+      for (size_t nu = 0; nu <= mu; ++nu) J[mu * n + nu] = J[nu * n + mu] = jp[mu * (mu + 1) / 2 + nu];
   }
 };
 
@@ -1303,11 +1312,13 @@ DFJ::DFJ(const BasisHost& orb, const BasisHost& aux, int dev) : device(dev), nbf
   std::vector<DevBuf<double>> od, xd;
   const DFBasisView ov = view(orb, oi, od, 2, "DF-J: orbital");
   const DFBasisView xv = view(aux, xi, xd, 4, "DF-J: auxiliary");
-  const size_t nn = (size_t)nbf * nbf;
-  B.alloc(nn * naux);
+  const size_t np = (size_t)nbf * (nbf + 1) / 2;
+  if (np > (size_t)std::numeric_limits<rocblas_int>::max())
+    throw Unsupported("DF-J: nbf (nbf + 1) / 2 exceeds the rocBLAS integer range");
+  B.alloc(np * naux);
   V.alloc((size_t)naux * naux);
-  d.alloc(nn);
-  j.alloc(nn);
+  d.alloc(np);
+  j.alloc(np);
   x.alloc(naux);
   launch_df_three_centre(ov, xv, tb.ra, tb.sph, B.p, nullptr);
   HIP_CHECK(hipGetLastError());
