@@ -346,13 +346,13 @@ void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
 // q = first pair of bucket kb + j), NK doubles each, from jbuf + jslot[item].
 // A tile is up to 256 consecutive bra pairs of one bra bucket (items
 // item0 + i * stride, i < np) against positions [j0, j0 + kJcdTileKet) of one
-// ket bucket whose first pair is q0.  One wave per tile: each lane owns one
-// position of a 64-wide column block, walks the tile's items (their prefix
-// lengths, slot bases and mask bases are loaded once, four per lane), and for
-// each 64-position block gathers every item's survivor word (one per lane),
-// visits only the items with a survivor there (two at a time, their bases
-// broadcast with readlane), reads the surviving shares with coalesced loads,
-// and adds the column sums to J.
+// ket bucket whose first pair is q0.  One wave per tile walks the tile's
+// items (their prefix lengths, slot bases and mask bases are loaded once, four
+// per lane); for each 64-position block it gathers every item's survivor word
+// (one per lane), visits only the items with a survivor there (two at a time,
+// their bases broadcast with readlane), reads the surviving shares of the
+// block as contiguous runs (lane l holds elements l, l + 64, ... of the
+// block's 64 * NK doubles) and adds the sums to J.
 template <int NK>
 __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tiles, int ntiles,
                                                    const int* __restrict__ nslot,
@@ -390,8 +390,10 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
     nmax = v > nmax ? v : nmax;
   }
   const int jend = tl.w + kJcdTileKet < nmax ? tl.w + kJcdTileKet : nmax;
+  // lanes own flattened elements of the 64-position block: element
+  // e = f * 64 + lane is function e % NK of position jc + e / NK, so a wave's
+  // share loads are contiguous 512-byte runs of the item's slots
   for (int jc = tl.w; jc < jend; jc += 64) {
-    const int j = jc + lane;
     double acc[NK];
 #pragma unroll
     for (int f = 0; f < NK; ++f) acc[f] = 0.0;
@@ -403,11 +405,13 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
       if (nl[k] > jc) wk = jmask[ml[k] + (jc >> 6)];
       unsigned long long act = __ballot(wk != 0ull);
       while (act) {
-        // two items per step, so that their share loads overlap
+        // two items per step, so that their share loads overlap (one for the
+        // largest ket pairs, whose shares would not fit the registers twice)
+        constexpr bool kTwo = NK <= 25;
         const int l0 = __builtin_ctzll(act);
         act &= act - 1ull;
-        const int l1 = act ? __builtin_ctzll(act) : l0;
-        if (act) act &= act - 1ull;
+        const int l1 = (kTwo && act) ? __builtin_ctzll(act) : l0;
+        if (kTwo && act) act &= act - 1ull;
         const unsigned long long w0 = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wk >> 32), l0) << 32) |
                                       (unsigned)__builtin_amdgcn_readlane((int)(wk & 0xffffffffull), l0);
         const unsigned long long w1 = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wk >> 32), l1) << 32) |
@@ -416,23 +420,36 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
                              (unsigned)__builtin_amdgcn_readlane((int)(bl[k] & 0xffffffffLL), l0);
         const long long b1 = ((long long)__builtin_amdgcn_readlane((int)(bl[k] >> 32), l1) << 32) |
                              (unsigned)__builtin_amdgcn_readlane((int)(bl[k] & 0xffffffffLL), l1);
-        const bool t0 = (w0 >> lane) & 1ull;
-        const bool t1 = l1 != l0 && ((w1 >> lane) & 1ull);
-        double v0[NK], v1[NK];
+        const double* s0 = jbuf + b0 + (long long)jc * NK;
+        const double* s1 = jbuf + b1 + (long long)jc * NK;
+        if constexpr (kTwo) {
+          double v0[NK], v1[NK];
 #pragma unroll
-        for (int f = 0; f < NK; ++f) {
-          v0[f] = t0 ? jbuf[b0 + (long long)j * NK + f] : 0.0;
-          v1[f] = t1 ? jbuf[b1 + (long long)j * NK + f] : 0.0;
+          for (int f = 0; f < NK; ++f) {
+            const int e = f * 64 + lane, jj = e / NK;
+            const bool t0 = (w0 >> jj) & 1ull;
+            const bool t1 = l1 != l0 && ((w1 >> jj) & 1ull);
+            v0[f] = t0 ? s0[e] : 0.0;
+            v1[f] = t1 ? s1[e] : 0.0;
+          }
+#pragma unroll
+          for (int f = 0; f < NK; ++f) acc[f] += v0[f] + v1[f];
+        } else {
+#pragma unroll
+          for (int f = 0; f < NK; ++f) {
+            const int e = f * 64 + lane;
+            if ((w0 >> (e / NK)) & 1ull) acc[f] += s0[e];
+          }
         }
-#pragma unroll
-        for (int f = 0; f < NK; ++f) acc[f] += v0[f] + v1[f];
       }
     }
-    if (j < nmax) {
-      const PairHdr h = pairs[tl.z + j];
 #pragma unroll
-      for (int f = 0; f < NK; ++f)
-        if (acc[f] != 0.0) unsafeAtomicAdd(&J[(size_t)(h.bfa + f / nsb) * nbf + h.bfb + f % nsb], acc[f]);
+    for (int f = 0; f < NK; ++f) {
+      const int e = f * 64 + lane, j = jc + e / NK, fn = e % NK;
+      if (j < nmax && acc[f] != 0.0) {
+        const PairHdr* h = pairs + tl.z + j;
+        unsafeAtomicAdd(&J[(size_t)(h->bfa + fn / nsb) * nbf + h->bfb + fn % nsb], acc[f]);
+      }
     }
   }
 }
