@@ -155,7 +155,7 @@ void qfe_dfj_destroy(qfe_dfj* h);
 int qfe_quarter_trans(qfe_handle* h, unsigned long long nt, const double* C, double* out);
 
 /* Four-index MO transform, MOERI::compute(nb, nt, Ci, Cj, Ck, Cl, out)
- * (moeri.cpp:28-200): out[w][x][y][z] = sum (ab|cd) Cl[a][w] Ck[b][x]
+ * (moeri.cpp:28-153): out[w][x][y][z] = sum (ab|cd) Cl[a][w] Ck[b][x]
  * Cj[c][y] Ci[d][z], each C row-major nb x nt, out nt^4 (host).  nb must be
  * the basis size; nt > nb returns QFE_ERR_RUNTIME with the reference's
  * message.  Four rocBLAS GEMMs over the dense tensor in HBM. */

@@ -209,9 +209,9 @@ def quarter_trans(T, C):
 
 
 def mo_eri(T, Ci, Cj, Ck, Cl):
-    """The four-quarter transform of MOERI::compute (moeri.cpp:28-200, CPU
-    path :118-198): tmp1 = quarter_trans(Ci); the 2nd quarter contracts the
-    third AO index with Cj (:124-133), the 3rd the second with Ck (:135-144),
+    """The four-quarter transform of MOERI::compute (moeri.cpp:28-153, CPU
+    path :108-152): tmp1 = quarter_trans(Ci); the 2nd quarter contracts the
+    third AO index with Cj (:125-133), the 3rd the second with Ck (:135-144),
     the 4th the first with Cl (:146-151).  Row-major result
     out[w, x, y, z] = sum (ab|cd) Cl[a, w] Ck[b, x] Cj[c, y] Ci[d, z]."""
     t1 = quarter_trans(T, Ci)                           # [a, b, c, z]

@@ -1339,7 +1339,7 @@ DFJ::DFJ(const BasisHost& orb, const BasisHost& aux, int dev) : device(dev), nbf
 // ------------------------------------------------------------ MO transforms
 // The dense AO tensor T[a][b][c][d] = (ab|cd) in HBM (nbf^4 doubles, shells
 // up to d), for the quarter transform (ERI::quarter_trans, eri.h:116) and
-// MOERI::compute (moeri.cpp:28-200).  Small systems only: nbf = 200 is
+// MOERI::compute (moeri.cpp:28-153).  Small systems only: nbf = 200 is
 // 12.8 GB.
 static void dense_tensor(const BasisHost& b, int dev, DevBuf<double>& T, size_t extra_doubles) {
   one_body_device(dev);

@@ -188,7 +188,7 @@ class DFJ:
 
 
 class MOERI:
-    """Four-index MO transform over an engine (MOERI, moeri.cpp:28-200):
+    """Four-index MO transform over an engine (MOERI, moeri.cpp:28-153):
     compute(nb, nt, Ci, Cj, Ck, Cl) returns out[w, x, y, z] =
     sum (ab|cd) Cl[a, w] Ck[b, x] Cj[c, y] Ci[d, z], each C of shape (nb, nt);
     with one C for all four, the MO integrals (pq|rs)."""
