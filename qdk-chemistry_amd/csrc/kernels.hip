@@ -93,12 +93,13 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
   int c64 = 0, c32 = 0, npre = 0;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int q0 = bk.x; q0 < end; q0 += 64) {
-    // sorted by Q descending: if the first ket fails the Schwarz bound, all later do
-    if (Qp * Q[q0] < cut) break;
     const int q = q0 + lane;
+    const double Qq = q < end ? Q[q] : 0.0;
+    // sorted by Q descending: if the first ket (lane 0, q0 < end) fails the
+    // Schwarz bound, all later do
+    if (Qp * __shfl(Qq, 0) < cut) break;
     bool ok = false, lo = false, pre = false;
     if (q < end) {
-      const double Qq = Q[q];
       pre = !(Qp * Qq < cut);
       if (pre) {
         const int2 cd = psh[q];
