@@ -257,6 +257,38 @@ __global__ __launch_bounds__(256) void k_df_metric(DFBasisView x, RysArgs ra, co
     }
 }
 
+// d[mu (mu + 1) / 2 + nu] = D[mu][nu] + D[nu][mu] (mu > nu), D[mu][mu] on the
+// diagonal: the density in the packed layout of B (one thread per element)
+__global__ __launch_bounds__(256) void k_df_pack(const double* __restrict__ D, double* __restrict__ d, int n) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * (n + 1) / 2) return;
+  long long mu = (long long)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
+  while (mu * (mu + 1) / 2 > e) --mu;
+  while ((mu + 1) * (mu + 2) / 2 <= e) ++mu;
+  const long long nu = e - mu * (mu + 1) / 2;
+  d[e] = mu == nu ? D[mu * n + mu] : D[mu * n + nu] + D[nu * n + mu];
+}
+
+// J[mu][nu] = J[nu][mu] = j[mu (mu + 1) / 2 + nu]: one thread per element of
+// the full matrix, so the stores are coalesced
+__global__ __launch_bounds__(256) void k_df_unpack(const double* __restrict__ j, double* __restrict__ J, int n) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const long long r = e / n, c = e % n;
+  const long long hi = r > c ? r : c, lo = r > c ? c : r;
+  J[e] = j[hi * (hi + 1) / 2 + lo];
+}
+
+void launch_df_pack(const double* D, double* d, int n, hipStream_t s) {
+  const long long m = (long long)n * (n + 1) / 2;
+  if (m > 0) hipLaunchKernelGGL(k_df_pack, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, D, d, n);
+}
+
+void launch_df_unpack(const double* j, double* J, int n, hipStream_t s) {
+  const long long m = (long long)n * n;
+  if (m > 0) hipLaunchKernelGGL(k_df_unpack, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, j, J, n);
+}
+
 void launch_df_three_centre(const DFBasisView& o, const DFBasisView& x, const RysArgs& ra, const double* sph,
                             double* B, hipStream_t s) {
   const long long n = (long long)o.nsh * (o.nsh + 1) / 2 * x.nsh;

@@ -25,6 +25,10 @@ struct DFBasisView {
 // solid-harmonic table kSph (device)
 void launch_df_three_centre(const DFBasisView& orb, const DFBasisView& aux, const RysArgs& ra, const double* sph,
                             double* B, hipStream_t s);
+// the density D (n x n, device) in the packed layout of B, and the packed J
+// back to the full symmetric matrix
+void launch_df_pack(const double* D, double* d, int n, hipStream_t s);
+void launch_df_unpack(const double* j, double* J, int n, hipStream_t s);
 // V[P][Q] = (P | Q), naux x naux
 void launch_df_metric(const DFBasisView& aux, const RysArgs& ra, const double* sph, double* V, hipStream_t s);
 

@@ -1023,7 +1023,7 @@ static void one_body_device(int device_id);
 // density packed as D_mu,nu + D_nu,mu below the diagonal and D_mu,mu on it.
 struct DFJ {
   int device = 0, nbf = 0, naux = 0;
-  DevBuf<double> B, V, d, x, j;
+  DevBuf<double> B, V, d, x, j, full;
   rocblas_handle blas = nullptr;
 
   static DFBasisView view(const BasisHost& b, std::vector<DevBuf<int>>& ib, std::vector<DevBuf<double>>& db,
@@ -1056,13 +1056,11 @@ struct DFJ {
   void build_j(const double* P, double* J) {
     HIP_CHECK(hipSetDevice(device));
     const size_t n = (size_t)nbf, np = n * (n + 1) / 2;
-    // D packed like B: D_mu,nu + D_nu,mu below the diagonal, D_mu,mu on it
-    std::vector<double> dp(np), jp(np);
-    for (size_t mu = 0; mu < n; ++mu) {
-      for (size_t nu = 0; nu < mu; ++nu) dp[mu * (mu + 1) / 2 + nu] = P[mu * n + nu] + P[nu * n + mu];
-      dp[mu * (mu + 1) / 2 + mu] = P[mu * n + mu];
-    }
-    HIP_CHECK(hipMemcpy(d.p, dp.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+    // D packed like B on the device: D_mu,nu + D_nu,mu below the diagonal,
+    // D_mu,mu on it (full is the n x n staging square, for D and then J)
+    HIP_CHECK(hipMemcpy(full.p, P, sizeof(double) * n * n, hipMemcpyHostToDevice));
+    launch_df_pack(full.p, d.p, nbf, nullptr);
+    HIP_CHECK(hipGetLastError());
     const double one = 1.0, zero = 0.0;
     // B is row-major [naux][np], i.e. column-major np x naux
     if (rocblas_dgemv(blas, rocblas_operation_transpose, (rocblas_int)np, naux, &one, B.p, (rocblas_int)np, d.p, 1,
@@ -1073,9 +1071,9 @@ struct DFJ {
     if (rocblas_dgemv(blas, rocblas_operation_none, (rocblas_int)np, naux, &one, B.p, (rocblas_int)np, x.p, 1, &zero,
                       j.p, 1) != rocblas_status_success)
       throw DeviceError("DF-J: rocblas_dgemv failed");
-    HIP_CHECK(hipMemcpy(jp.data(), j.p, sizeof(double) * np, hipMemcpyDeviceToHost));
-    for (size_t mu = 0; mu < n; ++mu)
-      for (size_t nu = 0; nu <= mu; ++nu) J[mu * n + nu] = J[nu * n + mu] = jp[mu * (mu + 1) / 2 + nu];
+    launch_df_unpack(j.p, full.p, nbf, nullptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(J, full.p, sizeof(double) * n * n, hipMemcpyDeviceToHost));
   }
 };
 
@@ -1319,6 +1317,7 @@ DFJ::DFJ(const BasisHost& orb, const BasisHost& aux, int dev) : device(dev), nbf
   V.alloc((size_t)naux * naux);
   d.alloc(np);
   j.alloc(np);
+  full.alloc((size_t)nbf * nbf);
   x.alloc(naux);
   launch_df_three_centre(ov, xv, tb.ra, tb.sph, B.p, nullptr);
   HIP_CHECK(hipGetLastError());
