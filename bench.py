@@ -9,10 +9,15 @@ RHF density, with the density and J/K resident in HBM.  Before the timed
 region the RHF is converged in FP64 (plugin defaults: convergence 1e-7,
 eri_threshold 1e-12, shell_pair_threshold 1e-12); with mixed precision the
 same SCF is repeated with FP32 for quartets whose Schwarz bound is below 1e-6
-and the energy difference is reported.  The CPU baseline is the oracle (a C
-OpenMP restatement of the reference's LIBINT2_DIRECT loop with its own
-Obara-Saika integrals, not Libint2) on a strided sample of bra shells of the
-same build.  Prints one JSON line.
+and the energy difference is reported.  The same builds are also timed at the
+atomic-guess density, and one ERI nuclear gradient (the adjoint kernel) at
+the converged density.  The headline builds use the default use_atomics =
+False (bitwise-repeatable fixed-point sums, the reference's default); the
+FP64 builds are repeated with use_atomics = True.  The CPU baseline is the oracle (a C OpenMP
+restatement of the reference's LIBINT2_DIRECT loop with its own Obara-Saika
+integrals, not Libint2) on a strided sample of bra shells of the same builds,
+at both densities, in a child process with the box's CPU share.  Prints one
+JSON line.
 """
 from __future__ import annotations
 
@@ -68,13 +73,18 @@ def timed_builds(eng, P, steps, warmup):
                 if nq > 0:
                     acc = per_class.setdefault((prec, cls), [0.0, 0.0, 0.0, 0])
                     acc[0] += nq
-                    acc[1] += npq
                     acc[2] += ms
                     acc[3] += 1
         st = eng.stats()
         quartets += st["quartets"]
         quartets32 += st["quartets_fp32"]
     dt = time.perf_counter() - t0
+    # the evaluated primitive quartets (the same in every build of one
+    # density), counted after the timed region
+    for prec in (0, 1):
+        for cls, nq, npq, ms in eng.class_stats(prec, executed=True):
+            if (prec, cls) in per_class:
+                per_class[(prec, cls)][1] = npq * per_class[(prec, cls)][3]
     n = P.shape[0]
     return dict(seconds=dt, quartets=quartets, quartets_fp32=quartets32, per_class=per_class, stats=eng.stats(),
                 J=eng.download(dJ, (n, n)), K=eng.download(dK, (n, n)))
@@ -133,22 +143,24 @@ def traffic_from_profile(kernel_tag):
 
 def log_classes(run, steps, top=12):
     """Per-class table on stderr: quartets, ms and TFLOP/s per build, and the
-    number of global FP64 atomics the digestion issues (J_cd + four K blocks,
-    J_ab wave-reduced and not counted)."""
+    exchange updates the digestion adds into the LDS rows of its segment
+    (four K blocks per shell quartet: (na + nb) x (nc + nd) LDS atomics; the
+    global atomics are the flushes of those rows, once per non-zero entry and
+    segment, the J_ab block sums and the J_cd tile sums)."""
     from qdk_chemistry_amd.flops import class_flops, nsph
     rows = []
     for (prec, cls), (nq, npq, ms, n) in run["per_class"].items():
         fp, fc = class_flops(*cls)
         na, nb, nc, nd = (nsph(x) for x in cls)
-        atom = nq * (nc * nd + na * nc + na * nd + nb * nc + nb * nd)
+        upd = nq * (na + nb) * (nc + nd)
         rows.append((ms / steps, prec, cls, nq / steps, npq / steps, (npq * fp + nq * fc) / (ms * 1e-3) / 1e12,
-                     atom / steps))
+                     upd / steps))
     rows.sort(reverse=True)
-    tot_at = sum(r[6] for r in rows)
-    log(f"classes by time (of {len(rows)}); global FP64 atomics per build {tot_at:.3g}")
-    for ms, prec, cls, nq, npq, tf, at in rows[:top]:
+    tot = sum(r[6] for r in rows)
+    log(f"classes by time (of {len(rows)}); exchange LDS updates per build {tot:.3g}")
+    for ms, prec, cls, nq, npq, tf, up in rows[:top]:
         log(f"  {'f32' if prec else 'f64'} ({cls[0]}{cls[1]}|{cls[2]}{cls[3]}) {ms:8.2f} ms  quartets {nq:.3g}  "
-            f"prim {npq:.3g}  {tf:6.2f} TFLOP/s  atomics {at:.3g}")
+            f"prim {npq:.3g}  {tf:6.2f} TFLOP/s  LDS updates {up:.3g}")
 
 
 def harness_timings(basis, mol, P, host=True):
@@ -250,18 +262,37 @@ def dfj_timings(nwat):
     return out
 
 
-def cpu_baseline(basis, mol, P, stride):
-    from oracle.oracle import OracleERI
-    threads = min(16, os.cpu_count() or 1)
-    o = OracleERI(basis, mol, nthreads=threads)
-    t0 = time.perf_counter()
-    o.build_JK_sample(P, stride, 0)
-    dt = time.perf_counter() - t0
-    q = o.last_stats["quartets"]
-    return dict(value=q / dt, unit="quartets/s", cores=threads, kind="port",
+def cpu_baseline(system_name, densities, stride, repeats=5):
+    """The oracle's build_JK on a bounded sample of the same build, run in a
+    child process (oracle/cpu_baseline.py) so that OMP_NUM_THREADS and
+    OMP_PROC_BIND=close reach its OpenMP runtime: the box's CPU share (the
+    OMP_NUM_THREADS it sets, else every core), one warm-up and the median of
+    `repeats` timed builds per density."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as tmp:
+        args = [sys.executable, os.path.join(REPO, "oracle", "cpu_baseline.py"), "--system", system_name,
+                "--stride", str(stride), "--repeats", str(repeats)]
+        for label, P in densities.items():
+            path = os.path.join(tmp, f"{label}.npy")
+            np.save(path, P)
+            args += ["--density", f"{label}={path}"]
+        r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed: {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    head = res["densities"]["converged"] if "converged" in res["densities"] else next(iter(res["densities"].values()))
+    dens = "; ".join(f"{k}: {v['quartets']:.0f} quartets, median {v['median_s']:.2f} s, {v['quartets_per_s']:.4g}/s"
+                     for k, v in res["densities"].items())
+    return dict(value=head["quartets_per_s"], unit="quartets/s", cores=res["cores"], kind="port",
+                cpu_model=res["cpu_model"], omp_proc_bind=res["omp_proc_bind"],
                 sample=f"oracle build_JK (C/OpenMP restatement of LIBINT2_DIRECT, Obara-Saika integrals, "
-                       f"not Libint2) over bra shells s1 % {stride} == 0 of the same build: "
-                       f"{int(q)} quartets in {dt:.1f} s")
+                       f"not Libint2) over the bra shells s1 % {stride} == 0 of the same build, 1 warm-up + "
+                       f"median of {repeats}, {res['cores']} threads on {res['cpu_model']}: {dens}",
+                densities=res["densities"])
 
 
 def main():
@@ -273,7 +304,7 @@ def main():
     ap.add_argument("--no-scf", action="store_true", help="time at the atomic-guess density")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=24)
+    ap.add_argument("--cpu-stride", type=int, default=96)
     ap.add_argument("--mo-waters", type=int, default=4, help="water cluster of the MO-transform timing (0: skip)")
     ap.add_argument("--mo-nt", type=int, default=48)
     ap.add_argument("--mo-only", action="store_true", help="only the MO-transform timing, one JSON line")
@@ -374,6 +405,44 @@ def main():
             ok = True
         if ok and s32["ms_per_build"] < s64["ms_per_build"]:
             headline, path = s32, "mixed"
+    # the same builds at the atomic-guess density (the first SCF iteration's
+    # full density; the screening keeps more quartets there)
+    guess = None
+    if "scf_fp64" in result:
+        Pg = atom_guess(mol, basis_name)
+        rg = timed_builds(eng, Pg, args.steps, args.warmup)
+        sg = summarize(rg, args.steps)
+        guess = dict(fp64={k: v for k, v in sg.items() if k != "dominant"})
+        log(f"atomic-guess density, FP64: {sg['ms_per_build']:.2f} ms/build, {sg['quartets_per_s']:.4g} quartets/s "
+            f"({sg['quartets_per_build']:.4g} quartets)")
+        if not args.no_mixed:
+            eng.set_precision(True, 1e-6)
+            rgm = timed_builds(eng, Pg, args.steps, args.warmup)
+            eng.set_precision(False)
+            sgm = summarize(rgm, args.steps)
+            guess["mixed"] = {k: v for k, v in sgm.items() if k != "dominant"}
+            guess["mixed"]["speedup_vs_fp64"] = sg["ms_per_build"] / sgm["ms_per_build"]
+            guess["mixed"]["max_abs_dF_vs_fp64"] = float(np.max(np.abs((rgm["J"] - 0.5 * rgm["K"]) -
+                                                                       (rg["J"] - 0.5 * rg["K"]))))
+            log(f"atomic-guess density, mixed: {sgm['ms_per_build']:.2f} ms/build "
+                f"({guess['mixed']['speedup_vs_fp64']:.2f}x)")
+    # ERI nuclear gradient (the adjoint kernel) at the same density
+    eng.get_gradients(P)
+    t = time.perf_counter()
+    eng.get_gradients(P)
+    gst = eng.stats()
+    grad = dict(seconds=time.perf_counter() - t, quartets=gst["quartets"], ms_screen=gst["ms_screen"],
+                ms_kernels=gst["ms_eri"])
+    log(f"ERI gradient: {grad['seconds'] * 1e3:.1f} ms ({gst['quartets']:.4g} quartets, kernels {gst['ms_eri']:.1f} ms)")
+    # the same FP64 builds with use_atomics = True (double-precision atomics:
+    # no fixed-point conversion, not bitwise repeatable), on a fresh engine
+    eng.close()
+    eng_a = create(basis, mol, ERIConfig(eri_threshold=1e-12, shell_pair_threshold=1e-12, profile=True,
+                                         use_atomics=True))
+    sa = summarize(timed_builds(eng_a, P, args.steps, args.warmup), args.steps)
+    eng_a.close()
+    atomics = {k: v for k, v in sa.items() if k != "dominant"}
+    log(f"use_atomics=True, FP64: {sa['ms_per_build']:.2f} ms/build, {sa['quartets_per_s']:.4g} quartets/s")
     out = {
         "metric": METRIC,
         "value": headline["quartets_per_s"],
@@ -389,7 +458,8 @@ def main():
         "data": f"synthetic seeded geometry, {density} density",
         "config": {"workload": f"RHF J/K build, {desc}, nbf={basis.nbf}, eri_threshold 1e-12",
                    "path": path, "fock_build_s": headline["ms_per_build"] / 1e3,
-                   "fp64": {k: v for k, v in s64.items() if k != "dominant"}, "mixed": mixed},
+                   "fp64": {k: v for k, v in s64.items() if k != "dominant"}, "mixed": mixed,
+                   "atom_guess_density": guess, "eri_gradient": grad, "fp64_use_atomics": atomics},
         "roofline": {"bound": "fp64-valu", "kernel": "ERI class (%d %d|%d %d), FP64" % cls,
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK,
@@ -405,9 +475,12 @@ def main():
     if args.mo_waters > 0:
         out["config"]["mo_transform"] = mo_transform_timings(args.mo_waters, args.mo_nt, not args.no_cpu)
     if not args.no_cpu:
-        cb = cpu_baseline(basis, mol, P, args.cpu_stride)
+        dens = {"converged": P} if "scf_fp64" in result else {"given": P}
+        if "scf_fp64" in result:
+            dens["atom_guess"] = Pg
+        cb = cpu_baseline(args.system, dens, args.cpu_stride)
         out["cpu_baseline"] = cb
-        log(f"cpu baseline: {cb['value']:.4g} quartets/s on {cb['cores']} threads")
+        log(f"cpu baseline: {cb['value']:.4g} quartets/s on {cb['cores']} threads ({cb['cpu_model']})")
     print(json.dumps(out), flush=True)
 
 

@@ -64,6 +64,8 @@ typedef struct {
   double mix_threshold;        /* 1e-6 */
   int verbose;
   int profile;                 /* 1: time every class kernel group with HIP events */
+  int use_atomics;             /* 0 (the reference's default, core/scf.h:81-82): bitwise-repeatable
+                                  J/K (order-independent fixed-point accumulation); 1: FP64 atomics */
 } qfe_config;
 
 /* Per-build statistics (the last build). */
@@ -81,8 +83,12 @@ typedef struct {
  * Replaces libint2::direct::ERI::ERI (libint2_direct.cpp:204-226). */
 int qfe_create(const qfe_basis* basis, const qfe_config* cfg, qfe_handle** out);
 
-/* J and K for ndm host densities.  Replaces ERI::build_JK
- * (eri.h:79-80; libint2_direct.cpp:256-626).  omega != 0 is unsupported. */
+/* J and K for ndm host densities (1: RHF total density; 2: the spin pair
+ * [P_alpha; P_beta], both screened with the norms of P_alpha as the
+ * reference's loop does, libint2_direct.cpp:271).  Replaces ERI::build_JK
+ * (eri.h:79-80; libint2_direct.cpp:256-626).  J or K may be NULL: that
+ * matrix is not built (ERIMultiplexer's split engines,
+ * eri_multiplexer.cpp:66-67).  omega != 0 is unsupported. */
 int qfe_build_jk(qfe_handle* h, const double* P, double* J, double* K, double alpha, double beta,
                  double omega, int ndm);
 
@@ -101,11 +107,15 @@ int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK,
 int qfe_device_buffers(qfe_handle* h, double** dP, double** dJ, double** dK);
 int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes, int kind);
 
-/* ERI nuclear-gradient contributions dJ, dK (3*natoms, xyz-major) for one
- * density.  Replaces ERI::get_gradients (eri.h:100-101;
- * libint2_direct.cpp:646-890). */
+/* ERI nuclear-gradient contributions dJ, dK (3*natoms, xyz-major) for ndm
+ * densities: 1 = the RHF total density; 2 = the spin pair [P_alpha; P_beta],
+ * whose Coulomb term takes P_alpha + P_beta and whose exchange term sums the
+ * spins with scale -1/2 (libint2_direct.cpp:683-692,866-886).  Quartets are
+ * screened with the norms of the first density block.  A NULL dJ or dK is
+ * not computed; the outputs are overwritten.  Replaces ERI::get_gradients
+ * (eri.h:100-101; libint2_direct.cpp:646-890). */
 int qfe_gradients(qfe_handle* h, const double* P, double* dJ, double* dK, double alpha,
-                  double beta, double omega);
+                  double beta, double omega, int ndm);
 
 /* One-electron matrices (host): op 0 = overlap, 1 = kinetic, 2 = nuclear
  * attraction.  Harness support for the SCF driver (scf/src/util/int1e.cpp). */
@@ -201,12 +211,20 @@ int qfe_get_stats(const qfe_handle* h, qfe_stats* out);
 /* Per-class census of the last build for precision 0 (FP64 tasks) or 1
  * (FP32 tasks of the mixed-precision path): for class pair i < min(cap, n):
  * cls[4i..4i+3] = (la, lb, lc, ld) as evaluated, quartets[i] and
- * prim_quartets[i] screened counts, ms[i] the kernel time of that class
- * (HIP events, only with qfe_config.profile; else 0).  Returns n (number of
- * class pairs), or -1 for a null handle.  No reference counterpart:
- * instrumentation. */
+ * prim_quartets[i] the screened shell quartets and their primitive quartets
+ * over significant primitive pairs, exec_prim_quartets[i] (may be NULL) the
+ * primitive quartets the kernels evaluated after primitive screening, and
+ * ms[i] the kernel time of that class (the last two only with
+ * qfe_config.profile, else 0).  Returns n (number of class pairs), or -1 for
+ * a null handle.  No reference counterpart: instrumentation. */
 int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, double* quartets,
-                    double* prim_quartets, double* ms);
+                    double* prim_quartets, double* exec_prim_quartets, double* ms);
+/* Chunk geometry of the J/K kernels of class (la lb | lc ld) as compiled:
+ * returns the number of chunks n and, for i < min(n, cap), out[4i..4i+3] =
+ * the chunk's bra functions [Q0, Q1) and ket cartesians [K0, K1); -1 for a
+ * class that is not compiled.  No reference counterpart: instrumentation, the
+ * input of the FLOP model behind the benchmark's roofline. */
+int qfe_class_chunks(int la, int lb, int lc, int ld, int cap, int* out);
 int qfe_nbf(const qfe_handle* h);
 /* Highest shell angular momentum compiled into the ERI kernels of this build
  * (qfe_create rejects bases with higher l).  No reference counterpart. */

@@ -15,7 +15,9 @@
  *                                   ordered reduction, symmetrisation)
  *   - ERI::get_gradients           libint2_direct.cpp:646-890  (ordered pairs,
  *                                   dJ = 1/2 sum P12 P34 d(12|34),
- *                                   dK = -1/4 sum P13 P24 d(12|34), xyz-major)
+ *                                   dK = -1/4 sum P13 P24 d(12|34), xyz-major;
+ *                                   spin pairs: P_alpha + P_beta for dJ,
+ *                                   -1/2 sum_s Ps13 Ps24 for dK)
  *   - INCORE J/K                   incore_impl.cpp:131-197 (full tensor)
  * The integrals themselves come from Libint2 in the reference (third party,
  * v2.9.0, not vendored).  Here they are Obara-Saika vertical recursion with
@@ -222,6 +224,7 @@ typedef struct {
   double *Q;           /* nsh x nsh Schwarz */
   int *splist_off;     /* nsh+1 offsets into splist */
   int *splist;         /* for each s1, significant s2 <= s1 (ascending) */
+  char *sig;           /* nsh x nsh: 1 for a significant pair (both orders) */
   int nthreads;
 } oracle_t;
 
@@ -814,7 +817,7 @@ static void compute_schwarz_and_pairs(oracle_t *h) {
           for (int x = 0; x < n1 * n2; ++x) f2 += ov[x] * ov[x];
           s = sqrt(f2) >= h->sp_thr;
         }
-        sig[(size_t)s1 * nsh + s2] = (char)s;
+        sig[(size_t)s1 * nsh + s2] = sig[(size_t)s2 * nsh + s1] = (char)s;
       }
     free(blk);
     work_free(&w);
@@ -831,7 +834,7 @@ static void compute_schwarz_and_pairs(oracle_t *h) {
       if (sig[(size_t)s1 * nsh + s2]) h->splist[cnt++] = s2;
   }
   h->splist_off[nsh] = cnt;
-  free(sig);
+  h->sig = sig;
 }
 
 /* Create from flat arrays.  prim_off[nsh+1] indexes exps/coefs. */
@@ -890,6 +893,7 @@ void oracle_destroy(oracle_t *h) {
   free(h->Q);
   free(h->splist_off);
   free(h->splist);
+  free(h->sig);
   free(h);
 }
 
@@ -1102,21 +1106,55 @@ int oracle_build_jk(const oracle_t *h, const double *P, double *J, double *K, do
 }
 
 /*
- * Per-class census of the screened canonical quartet list: for each class
- * (l1,l2,l3,l4) (index ((l1*5+l2)*5+l3)*5+l4), the surviving shell quartets
- * and their primitive quartets.  No integrals are computed.
+ * Primitive pairs of shells (s1, s2) that the reference's shell-pair data
+ * keeps: libint2::ShellPair is built with ln(max_engine_precision),
+ * max_engine_precision = DBL_EPSILON / 1e10 (libint2_direct.cpp:29,87,146),
+ * and drops a primitive pair whose |c_a c_b| exp(-a b / (a + b) |AB|^2) is
+ * below that precision.
  */
-void oracle_census(const oracle_t *h, const double *P, double *nq, double *nprim) {
+static int sig_prim_pairs(const oracle_t *h, int s1, int s2) {
+  const shell_t *A = &h->sh[s1], *B = &h->sh[s2];
+  double AB2 = 0.0;
+  for (int i = 0; i < 3; ++i) AB2 += (A->O[i] - B->O[i]) * (A->O[i] - B->O[i]);
+  const double cut = 2.220446049250313e-16 / 1e10;
+  int n = 0;
+  for (int i = 0; i < A->nprim; ++i)
+    for (int j = 0; j < B->nprim; ++j) {
+      const double a = A->a[i], b = B->a[j];
+      if (!(fabs(A->c[i] * B->c[j] * exp(-a * b / (a + b) * AB2)) < cut)) ++n;
+    }
+  return n;
+}
+
+/*
+ * Per-class census of the screened canonical quartet list: for each class
+ * (l1,l2,l3,l4) (index ((l1*5+l2)*5+l3)*5+l4), the surviving shell quartets,
+ * their primitive quartets (products of shell primitive counts) and, in
+ * nprim_sig (optional), their primitive quartets over the significant
+ * primitive pairs of sig_prim_pairs.  No integrals are computed.
+ */
+void oracle_census(const oracle_t *h, const double *P, double *nq, double *nprim, double *nprim_sig) {
   const int nsh = h->nsh;
   double *Pn = (double *)malloc(sizeof(double) * nsh * nsh);
   shellblock_norm(h, P, Pn);
+  /* significant primitive pairs per shell pair, only when asked for */
+  int *npp = NULL;
+  if (nprim_sig) {
+    npp = (int *)calloc((size_t)nsh * nsh, sizeof(int));
+#pragma omp parallel for schedule(dynamic) num_threads(h->nthreads)
+    for (int s1 = 0; s1 < nsh; ++s1)
+      for (int x12 = h->splist_off[s1]; x12 < h->splist_off[s1 + 1]; ++x12)
+        npp[(size_t)s1 * nsh + h->splist[x12]] = sig_prim_pairs(h, s1, h->splist[x12]);
+  }
   const int NC = 625;
   memset(nq, 0, sizeof(double) * NC);
   memset(nprim, 0, sizeof(double) * NC);
+  if (nprim_sig) memset(nprim_sig, 0, sizeof(double) * NC);
 #pragma omp parallel num_threads(h->nthreads)
   {
     double *lq = (double *)calloc(NC, sizeof(double));
     double *lp = (double *)calloc(NC, sizeof(double));
+    double *ls = (double *)calloc(NC, sizeof(double));
 #pragma omp for schedule(dynamic)
     for (int s1 = 0; s1 < nsh; ++s1)
       for (int x12 = h->splist_off[s1]; x12 < h->splist_off[s1 + 1]; ++x12) {
@@ -1134,6 +1172,8 @@ void oracle_census(const oracle_t *h, const double *P, double *nq, double *nprim
             int c = ((h->sh[s1].l * 5 + h->sh[s2].l) * 5 + h->sh[s3].l) * 5 + h->sh[s4].l;
             lq[c] += 1.0;
             lp[c] += (double)h->sh[s1].nprim * h->sh[s2].nprim * h->sh[s3].nprim * h->sh[s4].nprim;
+            if (npp)
+              ls[c] += (double)npp[(size_t)s1 * nsh + s2] * (double)npp[(size_t)s3 * nsh + s4];
           }
         }
       }
@@ -1141,10 +1181,13 @@ void oracle_census(const oracle_t *h, const double *P, double *nq, double *nprim
     for (int c = 0; c < NC; ++c) {
       nq[c] += lq[c];
       nprim[c] += lp[c];
+      if (nprim_sig) nprim_sig[c] += ls[c];
     }
     free(lq);
     free(lp);
+    free(ls);
   }
+  free(npp);
   free(Pn);
 }
 
@@ -1245,10 +1288,22 @@ static int class_key(int l1, int l2, int l3, int l4) {
 }
 
 int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK, double alpha,
-                     double beta, int mode) {
+                     double beta, int mode, int ndm) {
   const int nsh = h->nsh, n = h->nbf, na = h->natoms;
+  const size_t nn = (size_t)n * n;
   double *Pn = (double *)malloc(sizeof(double) * nsh * nsh);
+  /* screening with the first density block (libint2_direct.cpp:693) */
   shellblock_norm(h, P, Pn);
+  /* Coulomb density: the spin sum of a spin pair (:683-692) */
+  double *Ptot = NULL;
+  const double *PJ = P;
+  if (ndm == 2) {
+    Ptot = (double *)malloc(sizeof(double) * nn);
+    for (size_t x = 0; x < nn; ++x) Ptot[x] = P[x] + P[nn + x];
+    PJ = Ptot;
+  }
+  /* exchange scale of the ordered sum: -1/4 (one density), -1/2 per spin (:884) */
+  const double kfac = ndm == 2 ? -0.5 : -0.25;
   for (size_t x = 0; x < (size_t)nsh * nsh; ++x)
     if (!isfinite(Pn[x])) {
       free(Pn);
@@ -1310,24 +1365,28 @@ int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK,
           for (int j = 0; j < n2; ++j)
             for (int k = 0; k < n3; ++k)
               for (int l = 0; l < n4; ++l, ++x) {
-                vj += P[(size_t)(b1 + i) * n + b2 + j] * P[(size_t)(b3 + k) * n + b4 + l] * sh[x];
+                vj += PJ[(size_t)(b1 + i) * n + b2 + j] * PJ[(size_t)(b3 + k) * n + b4 + l] * sh[x];
                 /* ordered loop: P_ik P_jl; canonical loop: the permutation
-                 * average (P_ik P_jl + P_il P_jk) / 2 times the degeneracy */
-                const double pk = (mode == 0)
-                                      ? P[(size_t)(b1 + i) * n + b3 + k] * P[(size_t)(b2 + j) * n + b4 + l]
-                                      : 0.5 * (P[(size_t)(b1 + i) * n + b3 + k] * P[(size_t)(b2 + j) * n + b4 + l] +
-                                               P[(size_t)(b1 + i) * n + b4 + l] * P[(size_t)(b2 + j) * n + b3 + k]);
-                vk += pk * ks * sh[x];
+                 * average (P_ik P_jl + P_il P_jk) / 2 times the degeneracy;
+                 * summed over the spin densities */
+                for (int idm = 0; idm < ndm; ++idm) {
+                  const double *Pc = P + idm * nn;
+                  const double pk =
+                      (mode == 0) ? Pc[(size_t)(b1 + i) * n + b3 + k] * Pc[(size_t)(b2 + j) * n + b4 + l]
+                                  : 0.5 * (Pc[(size_t)(b1 + i) * n + b3 + k] * Pc[(size_t)(b2 + j) * n + b4 + l] +
+                                           Pc[(size_t)(b1 + i) * n + b4 + l] * Pc[(size_t)(b2 + j) * n + b3 + k]);
+                  vk += pk * ks * sh[x];
+                }
               }
         int coord = atoms[d / 3] + (d % 3) * na;
         if (g_grad_centre >= 0) {
           if (d / 3 != g_grad_centre) continue;
           int cd = atoms[3] + (d % 3) * na;
           lJ[cd] -= 0.5 * deg * vj;
-          lK[cd] -= -0.25 * deg * vk;
+          lK[cd] -= kfac * deg * vk;
         }
         lJ[coord] += 0.5 * deg * vj;
-        lK[coord] += -0.25 * deg * vk;
+        lK[coord] += kfac * deg * vk;
       }
     }
 #pragma omp critical(oracle_grad)
@@ -1345,6 +1404,134 @@ int oracle_gradients(const oracle_t *h, const double *P, double *dJ, double *dK,
   free(gJ);
   free(gK);
   free(op);
+  free(Pn);
+  free(Ptot);
+  return 0;
+}
+
+/*
+ * Exact rows of the J and K that build_JK returns, for the functions of the
+ * shells rows[0..nrows): Jr[f][nbf] and Kr[f][nbf] with f running over those
+ * shells' functions in the given order.  Every ordered quartet (ab|cd) with a
+ * a row shell, (a, b) and (c, d) significant pairs, is visited once and kept
+ * or skipped by the test the reference applies to its canonical
+ * representative: max6 * Q(s1, s2) * Q(s3, s4) < threshold with (s1, s2) the
+ * pair of larger (first, second) shell index (libint2_direct.cpp:357-402).
+ * Then J_ab = sum_cd (ab|cd) P_cd and K_ac = (alpha + beta) sum_bd (ab|cd)
+ * P_bd, which is what the canonical loop's degeneracy-weighted, symmetrised
+ * accumulation equals (libint2_direct.cpp:434-625).  One density; a cost of
+ * about (shells per row) x (significant pairs) quartets per row shell, so the
+ * rows of systems far too large for the full oracle stay cheap.
+ */
+int oracle_jk_rows(const oracle_t *h, const double *P, int nrows, const int *rows, double *Jr, double *Kr,
+                   double alpha, double beta) {
+  const int nsh = h->nsh, n = h->nbf;
+  double *Pn = (double *)malloc(sizeof(double) * nsh * nsh);
+  shellblock_norm(h, P, Pn);
+  for (size_t x = 0; x < (size_t)nsh * nsh; ++x)
+    if (!isfinite(Pn[x])) {
+      free(Pn);
+      return -1;
+    }
+  const double ks = alpha + beta;
+  int *roff = (int *)malloc(sizeof(int) * (nrows + 1));
+  roff[0] = 0;
+  for (int r = 0; r < nrows; ++r) roff[r + 1] = roff[r] + (h->bf0[rows[r] + 1] - h->bf0[rows[r]]);
+  const size_t nf = (size_t)roff[nrows];
+  memset(Jr, 0, sizeof(double) * nf * n);
+  memset(Kr, 0, sizeof(double) * nf * n);
+  /* ordered significant pairs (c, d) */
+  int nop = 0;
+  for (int c = 0; c < nsh; ++c)
+    for (int d = 0; d < nsh; ++d) nop += h->sig[(size_t)c * nsh + d];
+  int *op = (int *)malloc(sizeof(int) * 2 * (nop > 0 ? nop : 1));
+  nop = 0;
+  for (int c = 0; c < nsh; ++c)
+    for (int d = 0; d < nsh; ++d)
+      if (h->sig[(size_t)c * nsh + d]) {
+        op[2 * nop] = c;
+        op[2 * nop + 1] = d;
+        ++nop;
+      }
+  int lmax = 0;
+  for (int s = 0; s < nsh; ++s)
+    if (h->sh[s].l > lmax) lmax = h->sh[s].l;
+  /* work units: (row, b) with (a, b) significant */
+  int nunit = 0;
+  for (int r = 0; r < nrows; ++r)
+    for (int b = 0; b < nsh; ++b) nunit += h->sig[(size_t)rows[r] * nsh + b];
+  int *unit = (int *)malloc(sizeof(int) * 2 * (nunit > 0 ? nunit : 1));
+  nunit = 0;
+  for (int r = 0; r < nrows; ++r)
+    for (int b = 0; b < nsh; ++b)
+      if (h->sig[(size_t)rows[r] * nsh + b]) {
+        unit[2 * nunit] = r;
+        unit[2 * nunit + 1] = b;
+        ++nunit;
+      }
+  const int nth = h->nthreads;
+  double *Jl = (double *)calloc((size_t)nth * nf * n, sizeof(double));
+  double *Kl = (double *)calloc((size_t)nth * nf * n, sizeof(double));
+#pragma omp parallel num_threads(nth)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+    work_t w;
+    work_alloc(&w, lmax, 0);
+    size_t nmax = (size_t)nfun(h, lmax);
+    double *buf = (double *)malloc(sizeof(double) * nmax * nmax * nmax * nmax * 2 + 64);
+    double *Jt = Jl + (size_t)tid * nf * n, *Kt = Kl + (size_t)tid * nf * n;
+#pragma omp for schedule(dynamic, 1)
+    for (int u = 0; u < nunit; ++u) {
+      const int r = unit[2 * u], a = rows[r], b = unit[2 * u + 1];
+      const int ba = h->bf0[a], na_ = h->bf0[a + 1] - ba, bb = h->bf0[b], nb_ = h->bf0[b + 1] - bb;
+      const int p1 = a > b ? a : b, p2 = a > b ? b : a;
+      const double Qab = h->Q[(size_t)a * nsh + b];
+      for (int x = 0; x < nop; ++x) {
+        const int c = op[2 * x], d = op[2 * x + 1];
+        const int q1 = c > d ? c : d, q2 = c > d ? d : c;
+        const double Qcd = h->Q[(size_t)c * nsh + d];
+        const double pn = max6(Pn[(size_t)a * nsh + b], Pn[(size_t)a * nsh + c], Pn[(size_t)b * nsh + c],
+                               Pn[(size_t)a * nsh + d], Pn[(size_t)b * nsh + d], Pn[(size_t)c * nsh + d]);
+        const int bra_first = p1 > q1 || (p1 == q1 && p2 >= q2);
+        if (pn * (bra_first ? Qab : Qcd) * (bra_first ? Qcd : Qab) < h->eri_thr) continue;
+        const int bc = h->bf0[c], nc_ = h->bf0[c + 1] - bc, bd = h->bf0[d], nd_ = h->bf0[d + 1] - bd;
+        quartet_out(h, a, b, c, d, 0, &w, buf);
+        for (int i = 0, ijkl = 0; i < na_; ++i) {
+          double *Jrow = Jt + (size_t)(roff[r] + i) * n;
+          double *Krow = Kt + (size_t)(roff[r] + i) * n;
+          for (int j = 0; j < nb_; ++j) {
+            double jab = 0.0;
+            for (int k = 0; k < nc_; ++k) {
+              double kac = 0.0;
+              for (int l = 0; l < nd_; ++l, ++ijkl) {
+                const double v = buf[ijkl];
+                jab += v * P[(size_t)(bc + k) * n + bd + l];
+                kac += v * P[(size_t)(bb + j) * n + bd + l];
+              }
+              Krow[bc + k] += ks * kac;
+            }
+            Jrow[bb + j] += jab;
+          }
+        }
+      }
+    }
+    free(buf);
+    work_free(&w);
+  }
+  for (int t = 0; t < nth; ++t)
+    for (size_t x = 0; x < nf * n; ++x) {
+      Jr[x] += Jl[(size_t)t * nf * n + x];
+      Kr[x] += Kl[(size_t)t * nf * n + x];
+    }
+  free(Jl);
+  free(Kl);
+  free(unit);
+  free(op);
+  free(roff);
   free(Pn);
   return 0;
 }

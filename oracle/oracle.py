@@ -50,20 +50,23 @@ def lib():
         L.oracle_build_jk.restype = ctypes.c_int
         L.oracle_build_jk.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, _dp]
-        L.oracle_census.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.oracle_census.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.oracle_jk_incore.restype = ctypes.c_int
         L.oracle_eri_full.restype = ctypes.c_int
         L.oracle_eri_full.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_jk_incore.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.oracle_gradients.restype = ctypes.c_int
         L.oracle_gradients.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double,
-                                       ctypes.c_double, ctypes.c_int]
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.oracle_boys.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
         L.oracle_sph_row.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_set_row_sample.argtypes = [ctypes.c_int, ctypes.c_int]
         L.oracle_set_grad_class.argtypes = [ctypes.c_int]
         L.oracle_set_grad_centre.argtypes = [ctypes.c_int]
+        L.oracle_jk_rows.restype = ctypes.c_int
+        L.oracle_jk_rows.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, _dp, _dp, ctypes.c_double,
+                                     ctypes.c_double]
         _lib = L
     return _lib
 
@@ -155,12 +158,38 @@ class OracleERI:
         finally:
             lib().oracle_set_row_sample(1, 0)
 
-    def census(self, P):
+    def census(self, P, significant=False):
+        """Surviving shell quartets and their primitive quartets per class
+        [l1, l2, l3, l4] (the reference's loop orientation) of the screened
+        canonical loop (libint2_direct.cpp:357-402), screened with the norms
+        of P's first nbf x nbf block.  significant=True also returns the
+        primitive quartets over significant primitive pairs (|c_a c_b|
+        exp(-ab/p AB^2) >= DBL_EPSILON / 1e10, the precision the reference
+        builds its shell pairs with, :29,87,146)."""
         P = np.ascontiguousarray(P, dtype=np.float64)
         nq = np.zeros(625)
         npq = np.zeros(625)
-        lib().oracle_census(self.h, _p(P), _p(nq), _p(npq))
-        return nq.reshape(5, 5, 5, 5), npq.reshape(5, 5, 5, 5)
+        nsig = np.zeros(625)
+        lib().oracle_census(self.h, _p(P), _p(nq), _p(npq), _p(nsig) if significant else None)
+        out = (nq.reshape(5, 5, 5, 5), npq.reshape(5, 5, 5, 5))
+        return out + (nsig.reshape(5, 5, 5, 5),) if significant else out
+
+    def jk_rows(self, P, shells, alpha=1.0, beta=0.0):
+        """Exact rows of build_JK's (J, K) for the functions of `shells` (in
+        that order), each (nfunctions, nbf): every ordered quartet with a row
+        shell in the first position, screened as its canonical representative
+        is, so the rows equal those of the full screened build (one density)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        if P.shape != (self.nbf, self.nbf):
+            raise ValueError("jk_rows takes one nbf x nbf density")
+        rows = np.ascontiguousarray(shells, dtype=np.int32)
+        off = self.basis.shell_offsets
+        nf = int(sum(off[s + 1] - off[s] for s in rows))
+        J = np.zeros((nf, self.nbf))
+        K = np.zeros((nf, self.nbf))
+        if lib().oracle_jk_rows(self.h, _p(P), len(rows), _p(rows, ctypes.c_int), _p(J), _p(K), alpha, beta) != 0:
+            raise RuntimeError("Density matrix contains NaN/Inf values.")
+        return J, K
 
     def build_JK_incore(self, P, alpha=1.0, beta=0.0):
         P = np.ascontiguousarray(P, dtype=np.float64)
@@ -180,17 +209,19 @@ class OracleERI:
         return T
 
     def get_gradients(self, P, alpha=1.0, beta=0.0, mode=1, class_key=-1, centre=-1):
-        """(dJ, dK), each 3*natoms xyz-major (libint2_direct.cpp:824); class_key
+        """(dJ, dK), each 3*natoms xyz-major (libint2_direct.cpp:824), for the
+        RHF total density or a spin pair [P_alpha, P_beta]; class_key
         (1000 la + 100 lb + 10 lc + ld) restricts the sum to one ERI class,
         centre (0..2) to the derivative by that centre moved against centre D."""
         P = np.ascontiguousarray(P, dtype=np.float64)
+        ndm = P.size // (self.nbf * self.nbf)
         lib().oracle_set_grad_class(class_key)
         lib().oracle_set_grad_centre(centre)
         na = self.structure.natoms
         dJ = np.zeros(3 * na)
         dK = np.zeros(3 * na)
         try:
-            rc = lib().oracle_gradients(self.h, _p(P), _p(dJ), _p(dK), alpha, beta, mode)
+            rc = lib().oracle_gradients(self.h, _p(P), _p(dJ), _p(dK), alpha, beta, mode, ndm)
         finally:
             lib().oracle_set_grad_class(-1)
             lib().oracle_set_grad_centre(-1)

@@ -222,6 +222,15 @@ struct Engine {
   // size: tiles [begin, begin + count) of group g have nk, nsb = jt_groups[g]
   bool jcd_slots = true;
   int seg_min_waves = 8;
+  // deterministic builds (use_atomics = false): J and K summed as
+  // fixed-point integer pairs, so that repeated builds agree bitwise
+  bool det = true;
+  DevBuf<unsigned long long> Jx_d, Kx_d;
+  DevBuf<int> fxo_d;                 // fixed-point range overflow flag
+  // primitive-quartet screening at the engine precision (QFE_PRIM_SCREEN=0: off)
+  bool prim_screen = true;
+  int kcol_tile = 0;                 // QFE_KCOL_TILE: cap on the exchange-row tile width
+  std::vector<int> sh_bf0_h;         // first function of each shell (+ nbf), host
   std::vector<int> cls_pair_begin;  // first pair of each pair class (pairs are class-sorted)
   DevBuf<int> nslot_d, nmask_d;
   DevBuf<long long> jslot_d, mbase_d;
@@ -241,12 +250,14 @@ struct Engine {
 
   DevBuf<PairHdr> pairs_d;
   DevBuf<int2> psh_d;  // (sa, sb) per sorted pair, for the screening scan
-  DevBuf<float> Pcd_d;  // per pair: density norm of its shell block, per build
+  DevBuf<double> Pcd_d;  // per pair: density norm of its shell block, per build
   DevBuf<double4> ppos_d;
   DevBuf<double> pk_d;
   DevBuf<double> pa_d;      // exponent of each primitive pair's first shell (gradients)
   DevBuf<double> h2_d;      // 1/(2p) per primitive pair (p = exponent sum)
   DevBuf<float> h2_32_d;
+  DevBuf<double> sig_d;     // primitive-quartet screening factor per primitive pair
+  DevBuf<float> sig32_d;
   DevBuf<int> sh_atom_d;
   DevBuf<double> grad_d;    // [2][3 * natoms]
   DevBuf<float4> ppos32_d;  // single-precision copies for the FP32 tasks
@@ -258,17 +269,31 @@ struct Engine {
   DevBuf<int> item_cp_d;
   DevBuf<double> cstat_d;           // [4][ncp]: quartets, prim quartets (FP64), then (FP32)
   std::vector<double> cstat_h;
+  // executed primitive quartets per class pair [2][ncp] (profiled engines:
+  // the census minus those the primitive screening skips), counted on demand
+  // over the last J/K build's task list (last_bval: its class-pair bounds)
+  DevBuf<unsigned long long> pqc_d;
+  mutable std::vector<unsigned long long> pqc_h;
+  mutable bool pqc_valid = false;
+  std::vector<long long> last_bval;
   DevBuf<int> counts_d;
   DevBuf<long long> offsets_d;
   DevBuf<long long> bidx_d, bval_d;
   DevBuf<unsigned char> scan_tmp_d;
   size_t scan_tmp_bytes = 0;
   DevBuf<int2> tasks_d;
+  // the task list, J_cd share slots and survivor masks of the current build
+  // (the engine buffers above, or a caller's scratch)
+  int2* tasks_p = nullptr;
+  double* jbuf_p = nullptr;
+  unsigned long long* jmask_p = nullptr;
   DevBuf<int> sh_bf0_d, sh_nf_d;
-  DevBuf<float> Pn_d;
-  DevBuf<double> Pnd_d, pmax_d;
+  DevBuf<double> Pn_d, pmax_d;  // shell-block density norms (nsh x nsh) and their maximum
   DevBuf<int> bad_d;
-  DevBuf<double> P_d, J_d, K_d;
+  DevBuf<double> P_d, J_d, K_d;      // device buffers handed out by qfe_device_buffers (SCF context too)
+  DevBuf<double> hP_d, hJ_d, hK_d;   // staging of the host-pointer build (ndm x nbf x nbf)
+  DevBuf<double> gP_d;               // gradient densities: [P or P_alpha][P_beta][P_alpha + P_beta]
+  DevBuf<double> sph_d;              // solid-harmonic table kSph (runtime-l gradient kernel)
   std::vector<DevBuf<double>> rys_d;
   DevBuf<double> boys_d;
   std::vector<DevBuf<float>> rys32_d;
@@ -347,14 +372,30 @@ struct Engine {
     ra.boys = boys_d.p;
   }
 
-  // 1/(2p) of every primitive pair, in the order of the ppos table it goes with
-  void upload_h2(const std::vector<double4>& pp) {
-    std::vector<double> h(pp.size());
-    for (size_t i = 0; i < pp.size(); ++i) h[i] = 0.5 / pp[i].x;
+  // primitive-quartet screening factor of one primitive pair with exponent
+  // sum p and prefactor pk = c_a c_b exp(-ab/p AB^2) / p: ln(|pk| p^-1/4)
+  // + ln(2^1/2 pi^5/2) / 2, so that 2 pi^5/2 |pk_ab pk_cd| / sqrt(p + q) <=
+  // exp(sig_ab + sig_cd) (p + q >= 2 sqrt(pq))
+  static double prim_sig(double p, double pk) {
+    return std::log(std::fabs(pk)) - 0.25 * std::log(p) + 0.5 * std::log(std::sqrt(2.0) * 17.493418327624862);
+  }
+
+  // 1/(2p) and the screening factor of every primitive pair, in the order of
+  // the ppos / pk tables they go with
+  void upload_h2(const std::vector<double4>& pp, const std::vector<double>& pkv) {
+    std::vector<double> h(pp.size()), sg(pp.size());
+    for (size_t i = 0; i < pp.size(); ++i) {
+      h[i] = 0.5 / pp[i].x;
+      sg[i] = prim_sig(pp[i].x, pkv[i]);
+    }
     h2_d.upload(h, stream);
     h2_32_d.upload(to_float(h), stream);
+    sig_d.upload(sg, stream);
+    sig32_d.upload(to_float(sg), stream);
     ra.h2 = h2_d.p;
     ra32.h2 = h2_32_d.p;
+    ra.sig = sig_d.p;
+    ra32.sig = sig32_d.p;
   }
 
   void build_pairs() {
@@ -400,8 +441,10 @@ struct Engine {
         for (size_t ib = 0; ib < B.a.size(); ++ib) {
           const double a = A.a[ia], b = B.a[ib], p = a + b;
           const double K = A.c[ia] * B.c[ib] * std::exp(-a * b / p * AB2);
-          // primitive pairs below 1e-26 cannot contribute at double precision
-          if (std::fabs(K) < 1e-26) continue;
+          // the primitive pairs the reference's shell-pair data keeps: built
+          // at max_engine_precision = DBL_EPSILON / 1e10
+          // (libint2_direct.cpp:29,87,146)
+          if (std::fabs(K) < std::numeric_limits<double>::epsilon() / 1e10) continue;
           ppos.push_back(make_double4(p, (a * A.O[0] + b * B.O[0]) / p, (a * A.O[1] + b * B.O[1]) / p,
                                       (a * A.O[2] + b * B.O[2]) / p));
           pk.push_back(K / p);
@@ -418,7 +461,7 @@ struct Engine {
     pairs_d.upload(hdr, stream);
     ppos_d.upload(ppos, stream);
     pk_d.upload(pk, stream);
-    upload_h2(ppos);
+    upload_h2(ppos, pk);
     DevBuf<double> q2;
     q2.alloc(npair);
     HIP_CHECK(hipMemsetAsync(q2.p, 0, sizeof(double) * npair, stream));
@@ -468,7 +511,14 @@ struct Engine {
       PairHdr h = hdr[ord[i]];
       const int old0 = h.prim0;
       h.prim0 = (int)ppos2.size();
-      for (int k = 0; k < h.nprim; ++k) {
+      // primitives by screening factor, largest first: the primitive
+      // quartets a build keeps form a prefix of each pair's list
+      std::vector<int> po(h.nprim);
+      std::iota(po.begin(), po.end(), 0);
+      std::stable_sort(po.begin(), po.end(), [&](int x, int y) {
+        return prim_sig(ppos[old0 + x].x, pk[old0 + x]) > prim_sig(ppos[old0 + y].x, pk[old0 + y]);
+      });
+      for (int k : po) {
         ppos2.push_back(ppos[old0 + k]);
         pk2.push_back(pk[old0 + k]);
         pa2.push_back(pa[old0 + k]);
@@ -542,7 +592,7 @@ struct Engine {
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
     pa_d.upload(pa2, stream);
-    upload_h2(ppos2);
+    upload_h2(ppos2, pk2);
     {
       std::vector<float4> p32(ppos2.size());
       for (size_t i = 0; i < ppos2.size(); ++i)
@@ -561,6 +611,8 @@ struct Engine {
     item_cp_d.upload(item_cp, stream);
     cstat_d.alloc(4 * (size_t)ncp);
     cstat_h.assign(4 * (size_t)ncp, 0.0);
+    pqc_d.alloc(2 * (size_t)ncp);
+    pqc_h.assign(2 * (size_t)ncp, 0ull);
     const long long nitems = (long long)items_h.size();
     counts_d.alloc(2 * nitems + 1);
     HIP_CHECK(hipMemsetAsync(counts_d.p, 0, sizeof(int) * (2 * nitems + 1), stream));
@@ -667,6 +719,8 @@ struct Engine {
     nbf = basis.nbf;
     natoms = basis.natoms;
     if (nsh <= 0) throw std::invalid_argument("empty basis");
+    // the class kernels address the density with 32-bit byte offsets
+    if (nbf >= 23170) throw Unsupported("nbf >= 23170: density offsets exceed 32 bits");
     if (!basis.pure) throw Unsupported("cartesian (non-pure) bases are not compiled into this engine");
     for (const auto& s : basis.shells)
       if (s.l > QFE_MAX_L)
@@ -680,6 +734,7 @@ struct Engine {
       mix_thr = cfg->mix_threshold;
       verbose = cfg->verbose;
       profile = cfg->profile != 0;
+      det = cfg->use_atomics == 0;
     }
     int ndev = 0;
     HIP_CHECK(hipGetDeviceCount(&ndev));
@@ -695,6 +750,12 @@ struct Engine {
     // QFE_JCD_SLOTS=0 adds every task's J_cd share to J with global atomics
     if (const char* e = std::getenv("QFE_JCD_SLOTS"))
       if (e[0] == '0') jcd_slots = false;
+    // QFE_PRIM_SCREEN=0 evaluates every primitive quartet
+    if (const char* e = std::getenv("QFE_PRIM_SCREEN"))
+      if (e[0] == '0') prim_screen = false;
+    // QFE_KCOL_TILE=<columns>: exchange-row tiles of at most that many
+    // columns even where whole rows fit (exercises the tiled path)
+    if (const char* e = std::getenv("QFE_KCOL_TILE")) kcol_tile = std::max(0, std::atoi(e));
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     setup_rys();
@@ -705,11 +766,21 @@ struct Engine {
     }
     sh_bf0_d.upload(bf0, stream);
     sh_nf_d.upload(nf, stream);
+    sh_bf0_h = bf0;
+    sh_bf0_h.push_back(nbf);
+    fxo_d.alloc(1);
+    if (det) {  // the fixed-point sums of J and K, (hi, lo) per element
+      Jx_d.alloc(2 * (size_t)nbf * nbf);
+      Kx_d.alloc(2 * (size_t)nbf * nbf);
+    }
     std::vector<int> sha(nsh);
     for (int s2 = 0; s2 < nsh; ++s2) sha[s2] = basis.shells[s2].atom;
     sh_atom_d.upload(sha, stream);
+    {
+      const double* k0 = &::kSph[0][0][0];
+      sph_d.upload(std::vector<double>(k0, k0 + sizeof(::kSph) / sizeof(double)), stream);
+    }
     Pn_d.alloc((size_t)nsh * nsh);
-    Pnd_d.alloc((size_t)nsh * nsh);
     pmax_d.alloc(1);
     bad_d.alloc(1);
     build_pairs();
@@ -721,11 +792,22 @@ struct Engine {
     }
   }
 
+  // Device memory a build may take its per-build buffers from instead of the
+  // engine's own (the no-alloc entry point): base address and capacity.
+  struct Scratch {
+    unsigned char* base;
+    size_t bytes;
+  };
+
   // Screened canonical task list for density dP (FP64 and FP32 parts);
-  // returns the class-pair boundaries [2][ncp + 1] of the task array.
-  std::vector<long long> screen(const double* dP, hipStream_t s, bool allow_mix, bool slots = false) {
+  // returns the class-pair boundaries [2][ncp + 1] of the task array.  The
+  // task list, the J_cd share slots and their survivor masks go to tasks_p,
+  // jbuf_p and jmask_p: engine buffers grown as needed, or carved from ext,
+  // which must hold them (checked before any kernel writes there).
+  std::vector<long long> screen(const double* dP, hipStream_t s, bool allow_mix, bool slots = false,
+                                const Scratch* ext = nullptr) {
     HIP_CHECK(hipMemsetAsync(bad_d.p, 0, sizeof(int), s));
-    launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, Pnd_d.p, bad_d.p, pmax_d.p, s);
+    launch_shell_norm(dP, nbf, sh_bf0_d.p, sh_nf_d.p, nsh, Pn_d.p, bad_d.p, pmax_d.p, s);
     HIP_CHECK(hipGetLastError());
     const int npair = (int)pairs_h.size();
     launch_pair_norm(psh_d.p, npair, Pn_d.p, nsh, Pcd_d.p, s);
@@ -776,32 +858,72 @@ struct Engine {
     // against an independent sum before any kernel writes there
     if (total != sums[0] || total < 0 || (slots && (nslots != sums[1] || nslots < 0 || nwords < 0)))
       throw std::runtime_error("task-list scan totals disagree with their sums");
-    if (total > 0) tasks_d.alloc((size_t)total);
-    jslot_total = nslots;
-    if (slots && nslots > 0) {
-      jbuf_d.alloc((size_t)nslots);
-      jmask_d.alloc((size_t)nwords);
+    const bool use_slots = slots && nslots > 0;
+    if (ext) {
+      auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+      const size_t t_b = al(sizeof(int2) * (size_t)total);
+      const size_t m_b = use_slots ? al(sizeof(unsigned long long) * (size_t)nwords) : 0;
+      const size_t j_b = use_slots ? al(sizeof(double) * (size_t)nslots) : 0;
+      if (t_b + m_b + j_b > ext->bytes)
+        throw std::invalid_argument("the build needs " + std::to_string((t_b + m_b + j_b) >> 20) +
+                                    " MiB of task list and J share slots, more than the " +
+                                    std::to_string(ext->bytes >> 20) + " MiB scratch of the no-alloc entry point");
+      tasks_p = reinterpret_cast<int2*>(ext->base);
+      jmask_p = use_slots ? reinterpret_cast<unsigned long long*>(ext->base + t_b) : nullptr;
+      jbuf_p = use_slots ? reinterpret_cast<double*>(ext->base + t_b + m_b) : nullptr;
+    } else {
+      if (total > 0) tasks_d.alloc((size_t)total);
+      if (use_slots) {
+        jbuf_d.alloc((size_t)nslots);
+        jmask_d.alloc((size_t)nwords);
+      }
+      tasks_p = tasks_d.p;
+      jmask_p = use_slots ? jmask_d.p : nullptr;
+      jbuf_p = use_slots ? jbuf_d.p : nullptr;
     }
-    launch_task_write(sa, counts_d.p, offsets_d.p, tasks_d.p, mbase_d.p,
-                      (slots && nslots > 0) ? jmask_d.p : nullptr, s);
+    jslot_total = nslots;
+    launch_task_write(sa, counts_d.p, offsets_d.p, tasks_p, mbase_d.p, jmask_p, s);
     HIP_CHECK(hipGetLastError());
     return bval;
   }
 
-  // ERI gradient contributions (libint2_direct.cpp:646-890) for one density
-  void gradients(const double* P, double* dJ, double* dK, double kscale) {
+  // ERI gradient contributions (libint2_direct.cpp:646-890) for ndm host
+  // densities: ndm = 1 is the RHF total density; ndm = 2 the spin pair
+  // [P_alpha; P_beta], whose Coulomb term takes P_alpha + P_beta and whose
+  // exchange term sums the two spins with scale -1/2 (:683-692, :866-886).
+  // Quartets are screened with the norms of the first density block, as the
+  // reference's compute_shellblock_norm(obs_, P, nbf) reads (:693).  A null
+  // dJ or dK leaves that output untouched.
+  void gradients(const double* P, int ndm, double* dJ, double* dK, double kscale) {
+    last_bval.clear();  // the gradient's screening replaces the J/K task list
+    pqc_valid = false;
     if (natoms <= 0) throw std::invalid_argument("gradients need the atoms the shells sit on");
+    if (ndm < 1 || ndm > 2) throw std::invalid_argument("ndm must be 1 or 2");
     for (const auto& sh : basis.shells)
-      if (sh.l > 2) throw Unsupported("analytic ERI gradients are compiled for s, p and d shells only");
+      if (sh.l > grad_max_l())
+        throw Unsupported("analytic ERI gradients are compiled for shells up to l = " +
+                          std::to_string(grad_max_l()));
     HIP_CHECK(hipSetDevice(device));
     const size_t nn = (size_t)nbf * nbf;
-    P_d.alloc(nn);
     grad_d.alloc(6 * (size_t)natoms);
     hipStream_t s = stream;
-    HIP_CHECK(hipMemcpyAsync(P_d.p, P, nn * sizeof(double), hipMemcpyHostToDevice, s));
+    // device layout: [P_screen/P_alpha][P_beta][P_total]
+    gP_d.alloc(ndm == 1 ? nn : 3 * nn);
+    HIP_CHECK(hipMemcpyAsync(gP_d.p, P, ndm * nn * sizeof(double), hipMemcpyHostToDevice, s));
+    const double* Pj = gP_d.p;
+    const double* Pk = gP_d.p;
+    const double* Pk2 = nullptr;
+    if (ndm == 2) {
+      std::vector<double> tot(nn);
+      for (size_t i = 0; i < nn; ++i) tot[i] = P[i] + P[nn + i];
+      HIP_CHECK(hipMemcpyAsync(gP_d.p + 2 * nn, tot.data(), nn * sizeof(double), hipMemcpyHostToDevice, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      Pj = gP_d.p + 2 * nn;
+      Pk2 = gP_d.p + nn;
+    }
     HIP_CHECK(hipMemsetAsync(grad_d.p, 0, sizeof(double) * 6 * natoms, s));
     HIP_CHECK(hipEventRecord(ev[0], s));
-    std::vector<long long> bval = screen(P_d.p, s, false);
+    std::vector<long long> bval = screen(gP_d.p, s, false);
     HIP_CHECK(hipEventRecord(ev[1], s));
     const int ncp = (int)cp_bra.size();
     long long nq = 0;
@@ -809,8 +931,8 @@ struct Engine {
       const long long n = bval[cp + 1] - bval[cp];
       if (n <= 0) continue;
       nq += n;
-      GradLaunch a{tasks_d.p + bval[cp], n, pairs_d.p, ppos_d.p, pk_d.p, pa_d.p, &ra, P_d.p, nbf,
-                   sh_atom_d.p, natoms, kscale, grad_d.p, grad_d.p + 3 * natoms};
+      GradLaunch a{tasks_p + bval[cp], n, pairs_d.p, ppos_d.p, pk_d.p, pa_d.p, &ra, Pj, Pk, Pk2, nbf,
+                   sh_atom_d.p, natoms, kscale, grad_d.p, grad_d.p + 3 * natoms, sph_d.p};
       launch_grad(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
       HIP_CHECK(hipGetLastError());
     }
@@ -825,48 +947,41 @@ struct Engine {
     HIP_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
     stats.quartets = nq;
     stats.quartets_fp32 = 0;
+    double npq = 0.0;
+    for (int cp = 0; cp < ncp; ++cp) npq += cstat_h[ncp + cp];
+    stats.prim_quartets = (long long)npq;
     stats.ms_screen = t01;
     stats.ms_eri = t12;
     stats.ms_total = t01 + t12;
   }
 
-  // J/K for one density on device buffers
-  void build_device(const double* dP, double* dJ, double* dK, double kscale, hipStream_t s) {
-    HIP_CHECK(hipSetDevice(device));
-    const size_t nn = (size_t)nbf * nbf;
-    HIP_CHECK(hipEventRecord(ev[0], s));
+  // Class kernels and J_cd reduction of density dP over the task list of the
+  // last screen(): J += Coulomb (dJ null: skipped), K += exchange scaled by
+  // kscale (dK null: skipped).  J and K must be zeroed by the caller.
+  void digest(const std::vector<long long>& bval, const double* dP, double* dJ, double* dK, double kscale,
+              hipStream_t s, long long& nq, long long& nq32) {
     const int ncp = (int)cp_bra.size();
-    std::vector<long long> bval = screen(dP, s, true, jcd_slots);
-    HIP_CHECK(hipEventRecord(ev[1], s));
-    if (dJ) HIP_CHECK(hipMemsetAsync(dJ, 0, nn * sizeof(double), s));
-    if (dK) HIP_CHECK(hipMemsetAsync(dK, 0, nn * sizeof(double), s));
-    // J and K buffers are both required by the digestion kernels
-    double* Jw = dJ;
-    double* Kw = dK;
-    if (!Jw) {
-      J_d.alloc(nn);
-      Jw = J_d.p;
-      HIP_CHECK(hipMemsetAsync(Jw, 0, nn * sizeof(double), s));
-    }
-    if (!Kw) {
-      K_d.alloc(nn);
-      Kw = K_d.p;
-      HIP_CHECK(hipMemsetAsync(Kw, 0, nn * sizeof(double), s));
-    }
-    // J_cd shares in the items' slots, summed per ket pair after the class kernels
-    const bool jslots = jcd_slots && jslot_total > 0;
-    long long nq = 0, nq32 = 0;
+    const bool jslots = dJ && jcd_slots && jslot_total > 0;
+    if (!dK) kscale = 0.0;
+    // deterministic builds accumulate into the fixed-point pairs (zeroed by
+    // the caller) and leave dJ / dK to the finish kernel
+    unsigned long long* Jx = det && dJ ? Jx_d.p : nullptr;
+    unsigned long long* Kx = det && dK ? Kx_d.p : nullptr;
+    double* Jd = det ? nullptr : dJ;
+    nq = nq32 = 0;
     for (int prec = 0; prec < 2; ++prec) {
       const long long* b = bval.data() + prec * (ncp + 1);
       for (int cp = 0; cp < ncp; ++cp) {
         const long long n = b[cp + 1] - b[cp];
         if (n <= 0) continue;
         (prec ? nq32 : nq) += n;
-        JKLaunch a{tasks_d.p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jw, Kw, nbf, kscale,
+        JKLaunch a{tasks_p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jd, det ? nullptr : dK, nbf, kscale,
                    prec == 1, ppos32_d.p, pk32_d.p, &ra32,
                    segs_d.p + cp_seg_begin[cp], cp_seg_begin[cp + 1] - cp_seg_begin[cp],
-                   offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_d.p, lds_max,
-                   jslots ? jbuf_d.p : nullptr, jslot_d.p, qpos_d.p, seg_min_waves};
+                   offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_p, lds_max,
+                   jslots ? jbuf_p : nullptr, jslot_d.p, qpos_d.p, seg_min_waves,
+                   pmax_d.p, prim_screen ? 1 : 0, det ? 1 : 0, Jx, Kx, fxo_d.p, sh_bf0_h.data(), nsh,
+                   kcol_tile};
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
         if (e) HIP_CHECK(hipEventRecord(e[0], s));
         launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
@@ -876,16 +991,63 @@ struct Engine {
     }
     if (jslots) {
       for (const JTileGroup& g : jt_groups) {
-        launch_jcd_tiles(jtiles_d.p + g.begin, g.count, g.nk, g.nsb, nslot_d.p, jslot_d.p, mbase_d.p, jmask_d.p,
-                         jbuf_d.p, pairs_d.p, Jw, nbf, s);
+        launch_jcd_tiles(jtiles_d.p + g.begin, g.count, g.nk, g.nsb, nslot_d.p, jslot_d.p, mbase_d.p, jmask_p,
+                         jbuf_p, pairs_d.p, Jd, nbf, Jx, fxo_d.p, s);
         HIP_CHECK(hipGetLastError());
       }
     }
+  }
+
+  // J/K on device buffers for ndm densities dP[idm] (nbf x nbf each), all
+  // screened with the norms of dP[0] (libint2_direct.cpp:271 reads the first
+  // block only); dJ or dK may be null (that matrix is not built).
+  void build_device(const double* const* dP, double* const* dJ, double* const* dK, int ndm, double kscale,
+                    hipStream_t s, const Scratch* ext = nullptr) {
+    HIP_CHECK(hipSetDevice(device));
+    const size_t nn = (size_t)nbf * nbf;
+    const bool wantJ = dJ != nullptr && dJ[0] != nullptr;
+    const bool wantK = dK != nullptr && dK[0] != nullptr;
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    const int ncp = (int)cp_bra.size();
+    std::vector<long long> bval = screen(dP[0], s, true, wantJ && jcd_slots, ext);
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    long long nq = 0, nq32 = 0;
+    last_bval = bval;
+    pqc_valid = false;
+    if (det) {
+      if (wantJ) Jx_d.alloc(2 * nn);
+      if (wantK) Kx_d.alloc(2 * nn);
+      HIP_CHECK(hipMemsetAsync(fxo_d.p, 0, sizeof(int), s));
+    }
+    for (int idm = 0; idm < ndm; ++idm) {
+      double* Jw = wantJ ? dJ[idm] : nullptr;
+      double* Kw = wantK ? dK[idm] : nullptr;
+      if (det) {
+        if (Jw) HIP_CHECK(hipMemsetAsync(Jx_d.p, 0, 2 * nn * sizeof(unsigned long long), s));
+        if (Kw) HIP_CHECK(hipMemsetAsync(Kx_d.p, 0, 2 * nn * sizeof(unsigned long long), s));
+      } else {
+        if (Jw) HIP_CHECK(hipMemsetAsync(Jw, 0, nn * sizeof(double), s));
+        if (Kw) HIP_CHECK(hipMemsetAsync(Kw, 0, nn * sizeof(double), s));
+      }
+      digest(bval, dP[idm], Jw, Kw, kscale, s, nq, nq32);
+      if (det) launch_fixed_finish(Jx_d.p, Kx_d.p, Jw, Kw, nbf, 0.25 * kscale, s);
+    }
+    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ev[2], s));
-    launch_symmetrize(dJ, dK, nbf, s);
+    if (!det)
+      for (int idm = 0; idm < ndm; ++idm)
+        launch_symmetrize(wantJ ? dJ[idm] : nullptr, wantK ? dK[idm] : nullptr, nbf, s);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ev[3], s));
     HIP_CHECK(hipEventSynchronize(ev[3]));
+    if (det) {
+      int over = 0;
+      HIP_CHECK(hipMemcpyAsync(&over, fxo_d.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      if (over)
+        throw std::runtime_error("a J/K contribution left the range of the deterministic fixed-point sums "
+                                 "(|value| >= 2^20); build with use_atomics = true");
+    }
     float t01 = 0, t12 = 0, t03 = 0;
     HIP_CHECK(hipEventElapsedTime(&t01, ev[0], ev[1]));
     HIP_CHECK(hipEventElapsedTime(&t12, ev[1], ev[2]));
@@ -912,21 +1074,56 @@ struct Engine {
     stats.ms_total = t03;
   }
 
+  // the primitive quartets the class kernels evaluated in the last J/K build
+  // (profiled engines; zeros otherwise, or after a gradient replaced the
+  // task list): the same prefix walk, without the integrals, outside any
+  // timed build
+  void count_executed() const {
+    if (pqc_valid) return;
+    std::fill(pqc_h.begin(), pqc_h.end(), 0ull);
+    const int ncp = (int)cp_bra.size();
+    if (profile && last_bval.size() == 2 * (size_t)(ncp + 1)) {
+      HIP_CHECK(hipSetDevice(device));
+      HIP_CHECK(hipMemsetAsync(pqc_d.p, 0, sizeof(unsigned long long) * 2 * ncp, stream));
+      for (int prec = 0; prec < 2; ++prec) {
+        const long long* b = last_bval.data() + prec * (ncp + 1);
+        for (int cp = 0; cp < ncp; ++cp)
+          launch_count_prims(tasks_p + b[cp], b[cp + 1] - b[cp], pairs_d.p, sig_d.p, sig32_d.p, pmax_d.p,
+                             prim_screen ? 1 : 0, prec, pqc_d.p + (size_t)prec * ncp + cp, stream);
+      }
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(pqc_h.data(), pqc_d.p, sizeof(unsigned long long) * 2 * ncp, hipMemcpyDeviceToHost,
+                               stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    pqc_valid = true;
+  }
+
+  // one density on device buffers (the single-density form of build_device)
+  void build_device(const double* dP, double* dJ, double* dK, double kscale, hipStream_t s,
+                    const Scratch* ext = nullptr) {
+    const double* p[1] = {dP};
+    double* j[1] = {dJ};
+    double* k[1] = {dK};
+    build_device(p, j, k, 1, kscale, s, ext);
+  }
+
   void build_host(const double* P, double* J, double* K, double alpha, double beta, double omega, int ndm) {
     if (std::fabs(omega) > 1e-12)
       throw Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
     if (ndm < 1 || ndm > 2) throw std::invalid_argument("ndm must be 1 or 2");
     const size_t nn = (size_t)nbf * nbf;
-    P_d.alloc(nn);
-    J_d.alloc(nn);
-    K_d.alloc(nn);
-    for (int idm = 0; idm < ndm; ++idm) {
-      HIP_CHECK(hipMemcpyAsync(P_d.p, P + idm * nn, nn * sizeof(double), hipMemcpyHostToDevice, stream));
-      build_device(P_d.p, J_d.p, K_d.p, alpha + beta, stream);
-      if (J) HIP_CHECK(hipMemcpyAsync(J + idm * nn, J_d.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
-      if (K) HIP_CHECK(hipMemcpyAsync(K + idm * nn, K_d.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
-      HIP_CHECK(hipStreamSynchronize(stream));
-    }
+    hP_d.alloc(ndm * nn);
+    if (J) hJ_d.alloc(ndm * nn);
+    if (K) hK_d.alloc(ndm * nn);
+    HIP_CHECK(hipMemcpyAsync(hP_d.p, P, ndm * nn * sizeof(double), hipMemcpyHostToDevice, stream));
+    const double* p[2] = {hP_d.p, hP_d.p + nn};
+    double* j[2] = {J ? hJ_d.p : nullptr, J ? hJ_d.p + nn : nullptr};
+    double* k[2] = {K ? hK_d.p : nullptr, K ? hK_d.p + nn : nullptr};
+    build_device(p, J ? j : nullptr, K ? k : nullptr, ndm, alpha + beta, stream);
+    if (J) HIP_CHECK(hipMemcpyAsync(J, hJ_d.p, ndm * nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (K) HIP_CHECK(hipMemcpyAsync(K, hK_d.p, ndm * nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
   }
 };
 
@@ -1183,13 +1380,19 @@ int qfe_copy(qfe_handle* h, void* dst, const void* src, unsigned long long bytes
 }
 
 int qfe_gradients(qfe_handle* h, const double* P, double* dJ, double* dK, double alpha, double beta,
-                  double omega) {
+                  double omega, int ndm) {
   return guarded([&] {
     if (!h) throw std::runtime_error("ERI engine is not initialized");
     if (!P) throw std::invalid_argument("null density");
     if (std::fabs(omega) > 1e-12)
       throw qfe::Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
-    h->e.gradients(P, dJ, dK, alpha + beta);
+    if (dJ) std::fill_n(dJ, 3 * (size_t)std::max(h->e.natoms, 0), 0.0);
+    if (dK) std::fill_n(dK, 3 * (size_t)std::max(h->e.natoms, 0), 0.0);
+    // nothing requested, or an exchange term of zero weight alone
+    // (libint2_direct.cpp:657-680)
+    const bool need_k = dK != nullptr && std::fabs(alpha + beta) > 1e-12;
+    if (!dJ && !need_k) return;
+    h->e.gradients(P, ndm, dJ, need_k ? dK : nullptr, alpha + beta);
   });
 }
 
@@ -1210,9 +1413,17 @@ int qfe_get_stats(const qfe_handle* h, qfe_stats* out) {
 }
 
 int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, double* quartets,
-                    double* prim_quartets, double* ms) {
+                    double* prim_quartets, double* exec_prim_quartets, double* ms) {
   if (!h) return -1;
   const auto& e = h->e;
+  if (exec_prim_quartets) {
+    try {
+      e.count_executed();
+    } catch (const std::exception& ex) {
+      qfe::g_last_error = ex.what();
+      return -1;
+    }
+  }
   const int ncp = (int)e.cp_bra.size();
   for (int cp = 0; cp < ncp && cp < cap; ++cp) {
     const int b = e.cp_bra[cp], k = e.cp_ket[cp];
@@ -1229,6 +1440,8 @@ int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, doubl
     quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[o + cp];
     prim_quartets[cp] = e.cstat_h.empty() ? 0.0 : e.cstat_h[o + ncp + cp];
     if (ms) ms[cp] = e.cls_ms.empty() ? 0.0 : e.cls_ms[(precision ? ncp : 0) + cp];
+    if (exec_prim_quartets)
+      exec_prim_quartets[cp] = e.pqc_h.empty() ? 0.0 : (double)e.pqc_h[(precision ? ncp : 0) + cp];
   }
   return ncp;
 }

@@ -12,6 +12,7 @@
 #include <stdexcept>
 
 #include "eri_core.hpp"
+#include "fixed_point.hpp"
 #include "kernels.hpp"
 
 namespace qfe {
@@ -56,10 +57,10 @@ constexpr int kAccMax = 48;
 // QFE_JK_ACC_F (defined by its generated source).
 template <class C>
 constexpr int jk_acc() {
-  if (C::LA <= 2 && C::LB <= 2 && C::LC <= 2 && C::LD <= 2) return kAccMax;
 #ifdef QFE_JK_ACC_F
   return QFE_JK_ACC_F;
 #else
+  if (C::LA <= 2 && C::LB <= 2 && C::LC <= 2 && C::LD <= 2) return kAccMax;
   return C::L >= 11 ? 16 : 24;
 #endif
 }
@@ -112,8 +113,11 @@ QFE_DI double wave_sum(double v) {
 // K <- (K + K^T)/2 makes the placement immaterial), so the block keeps those
 // rows in LDS, adds into them with LDS atomics, and flushes each non-zero
 // entry once with one global atomic.  J_ab is reduced over the block; J_cd
-// (one entry per task) stays a global atomic.  With PROW the density rows of
-// a and b are staged in LDS beside the K rows.
+// (one entry per task) goes to the task's share slot in jbuf, or is a global
+// atomic when jbuf is null.  With PROW the density rows of a and b are staged
+// in LDS beside the K rows.  A null J skips the Coulomb digestion and
+// kscale = 0 the exchange digestion (build_JK with J or K null,
+// eri_multiplexer.cpp:66-67).
 template <class C, int Q0, int Q1>
 struct SegRows {
   static constexpr int MA0 = Q0 / C::NSB, MA1 = (Q1 - 1) / C::NSB;
@@ -123,6 +127,13 @@ struct SegRows {
   static constexpr int NMB = MB_HI - MB_LO + 1;
   static constexpr int NR = NMA + NMB;  // LDS rows: a functions, then b functions
 };
+
+// P[e] for an element index e < 2^29 (nbf < 23170, checked on the host): a
+// 32-bit byte offset from the uniform base, which the load can take in its
+// scalar-base form (no 64-bit address per lane)
+QFE_DI double pload(const double* __restrict__ P, unsigned e) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(P) + (e << 3));
+}
 
 template <class C, int Q0, int Q1, typename R, bool PROW, int K0, int K1, int BLK>
 __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ segs,
@@ -134,13 +145,18 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
                                                     const double* __restrict__ P, double* __restrict__ J,
                                                     double* __restrict__ K, int nbf, double kscale,
                                                     double* __restrict__ jbuf, const long long* __restrict__ jslot,
-                                                    const int2* __restrict__ qpos, int jfirst, int nsplit) {
+                                                    const int2* __restrict__ qpos, int jfirst, int nsplit,
+                                                    SegExtra ex) {
   using S = SegRows<C, Q0, Q1>;
   extern __shared__ double lds[];
-  // block = (segment, part): a long segment may be split over nsplit blocks,
-  // each with its own LDS rows, when there are too few segments for the GPU
-  const int2 sg = segs[blockIdx.x / nsplit];
-  const int part = (int)(blockIdx.x % nsplit);
+  // block = (segment, part, column tile): a long segment may be split over
+  // nsplit blocks, each with its own LDS rows, when there are too few
+  // segments for the GPU; the exchange rows may be split into column tiles
+  // when a whole row does not fit the LDS
+  const int tile = (int)(blockIdx.x % (unsigned)ex.ntile);
+  const int rest = (int)(blockIdx.x / (unsigned)ex.ntile);
+  const int2 sg = segs[rest / nsplit];
+  const int part = rest % nsplit;
   const long long s0 = offs[sg.x], s1 = offs[sg.y];
   const long long t0 = s0 + (s1 - s0) * part / nsplit, t1 = s0 + (s1 - s0) * (part + 1) / nsplit;
   if (t0 >= t1) return;  // the same for every thread of the block
@@ -148,27 +164,41 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
   constexpr int NQ = Q1 - Q0;
   const size_t n = (size_t)nbf;
   const int i0 = bra.bfa, j0 = bra.bfb;
+  // this block's exchange columns: column tiles only in the FP64 kernel with
+  // K rows only (FP32 tasks that need tiles are evaluated by it)
+  constexpr bool TILED = !PROW && sizeof(R) == 8;
+  const int c0 = TILED ? ex.tile_lo[tile] : 0, W = TILED ? ex.tile_hi[tile] - c0 : nbf;
+  const bool det = ex.det != 0;
+  const bool doJ = det ? ex.Jx != nullptr : J != nullptr;
+  const bool doK = kscale != 0.0;
+  // exchange rows: doubles, or (hi, lo) fixed-point pairs in deterministic builds
+  const size_t krow = det ? 2 * (size_t)W : (size_t)W;
   double* Ks = lds;
-  const double* Pa = PROW ? lds + (size_t)S::NR * nbf : nullptr;
-  __shared__ double sjab[NQ];  // J_ab of the block's bra functions
-  if (threadIdx.x < NQ) sjab[threadIdx.x] = 0.0;
+  unsigned long long* Kf = reinterpret_cast<unsigned long long*>(lds);
+  const double* Pa = PROW ? lds + (size_t)S::NR * krow : nullptr;
+  __shared__ unsigned long long sjab[2 * NQ];  // J_ab of the block's bra functions
+  for (int i = threadIdx.x; i < 2 * NQ; i += blockDim.x) sjab[i] = 0ull;
   for (int r = 0; r < S::NR; ++r) {
     const size_t ao = r < S::NMA ? (size_t)(i0 + S::MA0 + r) : (size_t)(j0 + S::MB_LO + (r - S::NMA));
-    for (int c = threadIdx.x; c < nbf; c += blockDim.x) {
-      Ks[(size_t)r * nbf + c] = 0.0;
-      if constexpr (PROW) lds[(size_t)(S::NR + r) * nbf + c] = P[ao * n + c];
-    }
+    for (int c = threadIdx.x; c < (int)krow; c += blockDim.x) Ks[(size_t)r * krow + c] = 0.0;
+    if constexpr (PROW)
+      for (int c = threadIdx.x; c < W; c += blockDim.x) lds[(size_t)S::NR * krow + (size_t)r * W + c] = P[ao * n + c0 + c];
   }
   __syncthreads();
   // density row of bra function (row r of the LDS layout)
   auto prow = [&](int r, int col) -> double {
-    if constexpr (PROW) {
-      return Pa[(size_t)r * nbf + col];
-    } else {
-      const size_t ao = r < S::NMA ? (size_t)(i0 + S::MA0 + r) : (size_t)(j0 + S::MB_LO + (r - S::NMA));
-      return P[ao * n + col];
-    }
+    if constexpr (PROW) return Pa[(size_t)r * W + col];
+    const unsigned ao = r < S::NMA ? (unsigned)(i0 + S::MA0 + r) : (unsigned)(j0 + S::MB_LO + (r - S::NMA));
+    return pload(P, ao * (unsigned)nbf + (unsigned)col);
   };
+  int bad = 0;  // a fixed-point add out of range (reported after the last add)
+  auto kadd = [&](int r, int col, double v) {
+    if (det)
+      fx_lds_add(Kf + (size_t)r * krow + 2 * (size_t)(col - c0), v, bad);
+    else
+      atomicAdd(&Ks[(size_t)r * krow + col - c0], v);
+  };
+  const R lnp = prim_ln_cut<R>(ex.pmax, ex.prim_screen);
   double pij[NQ], jij[NQ];
   sfor<NQ>([&](auto qq) {
     constexpr int qi = Q0 + qq;
@@ -179,14 +209,24 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
   for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int2 tk = tasks[t];
     const PairHdr ket = pairs[tk.y];
-    R acc[NQ][K1 - K0];
-    contract_chunk<C, Q0, Q1, R, K0, K1>(bra, ket, ppos, pk, ra, acc);
-    const double deg = dbra * (ket.sa == ket.sb ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
     const int k0 = ket.bfa, l0 = ket.bfb;
+    // with column tiles a task belongs to the tiles of its ket shells: its
+    // c-column updates and its Coulomb terms go with the tile of c, its
+    // d-column updates with the tile of d
+    const bool inC = !TILED || (unsigned)(k0 - c0) < (unsigned)W;
+    const bool inD = !TILED || (unsigned)(l0 - c0) < (unsigned)W;
+    if (!inC && !inD) continue;
+    R acc[NQ][K1 - K0];
+    contract_chunk<C, Q0, Q1, R, K0, K1>(bra, ket, ppos, pk, ra, acc, lnp);
+    // keep the digestion's density loads after the contraction (hoisted
+    // above it they hold registers through the whole primitive loop)
+    asm volatile("" ::: "memory");
+    const double deg = dbra * (ket.sa == ket.sb ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
+    const double degj = inC ? deg : 0.0;
     double Pkl[C::NKETS], jkl[C::NKETS];
     sfor<C::NSC>([&](auto c) {
       sfor<C::NSD>([&](auto d) {
-        Pkl[c * C::NSD + d] = P[(size_t)(k0 + c) * n + l0 + d];
+        Pkl[c * C::NSD + d] = pload(P, (unsigned)(k0 + c) * (unsigned)nbf + (unsigned)(l0 + d));
         jkl[c * C::NSD + d] = 0.0;
       });
     });
@@ -208,8 +248,9 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
         const double Pac = prow(ia, k0 + c), Pbc = prow(S::NMA + ib, k0 + c);
         sfor<C::NSD>([&](auto d) {
           const double x = v[c * C::NSD + d] * deg;
+          const double xj = v[c * C::NSD + d] * degj;
           const double Pbd = prow(S::NMA + ib, l0 + d), Pad = prow(ia, l0 + d);
-          jij[qq] = fma(Pkl[c * C::NSD + d], x, jij[qq]);
+          jij[qq] = fma(Pkl[c * C::NSD + d], xj, jij[qq]);
           jkl[c * C::NSD + d] = fma(pij[qq], x, jkl[c * C::NSD + d]);
           kac[ia][c] = fma(Pbd, x, kac[ia][c]);
           kbc[ib][c] = fma(Pad, x, kbc[ib][c]);
@@ -218,54 +259,79 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
         });
       });
     });
-    if (jbuf) {
-      // J_cd share of this task in its item's slots, summed per ket pair
-      // after the class kernels
-      const int2 qp = qpos[tk.y];
-      double* jt = jbuf + jslot[sg.x + qp.x] + (long long)qp.y * C::NKETS;
-      if (jfirst) {
-        sfor<C::NKETS>([&](auto f) { jt[f] = jkl[f]; });
+    if (doJ && inC) {
+      if (jbuf) {
+        // J_cd share of this task in its item's slots, summed per ket pair
+        // after the class kernels
+        const int2 qp = qpos[tk.y];
+        double* jt = jbuf + jslot[sg.x + qp.x] + (long long)qp.y * C::NKETS;
+        if (jfirst) {
+          sfor<C::NKETS>([&](auto f) { jt[f] = jkl[f]; });
+        } else {
+          sfor<C::NKETS>([&](auto f) { jt[f] += jkl[f]; });
+        }
       } else {
-        sfor<C::NKETS>([&](auto f) { jt[f] += jkl[f]; });
+        sfor<C::NSC>([&](auto c) {
+          sfor<C::NSD>([&](auto d) {
+            const size_t e = (size_t)(k0 + c) * n + l0 + d;
+            if (det)
+              fx_global_add(ex.Jx + 2 * e, jkl[c * C::NSD + d], bad);
+            else
+              unsafeAtomicAdd(&J[e], jkl[c * C::NSD + d]);
+          });
+        });
       }
-    } else {
-      sfor<C::NSC>([&](auto c) {
-        sfor<C::NSD>([&](auto d) { unsafeAtomicAdd(&J[(size_t)(k0 + c) * n + l0 + d], jkl[c * C::NSD + d]); });
-      });
     }
-    if (kscale != 0.0) {
+    if (doK) {
       sfor<S::NMA>([&](auto a) {
-        double* row = Ks + (size_t)a * nbf;
-        sfor<C::NSC>([&](auto c) { atomicAdd(&row[k0 + c], kac[a][c]); });
-        sfor<C::NSD>([&](auto d) { atomicAdd(&row[l0 + d], kad[a][d]); });
+        if (inC) sfor<C::NSC>([&](auto c) { kadd(a, k0 + c, kac[a][c]); });
+        if (inD) sfor<C::NSD>([&](auto d) { kadd(a, l0 + d, kad[a][d]); });
       });
       sfor<S::NMB>([&](auto b) {
-        double* row = Ks + (size_t)(S::NMA + b) * nbf;
-        sfor<C::NSC>([&](auto c) { atomicAdd(&row[k0 + c], kbc[b][c]); });
-        sfor<C::NSD>([&](auto d) { atomicAdd(&row[l0 + d], kbd[b][d]); });
+        if (inC) sfor<C::NSC>([&](auto c) { kadd(S::NMA + b, k0 + c, kbc[b][c]); });
+        if (inD) sfor<C::NSD>([&](auto d) { kadd(S::NMA + b, l0 + d, kbd[b][d]); });
       });
     }
   }
   // J_ab: every thread of the block holds partial sums of the same entries;
-  // waves reduce in registers, then through LDS to one global atomic each
+  // waves reduce in registers (a fixed order), then through LDS to one global
+  // atomic each
   const int lane = (int)(threadIdx.x & 63);
-  sfor<NQ>([&](auto qq) {
-    const double tot = wave_sum(jij[qq]);
-    if (lane == 0 && tot != 0.0) atomicAdd(&sjab[qq], tot);
-  });
-  __syncthreads();
-  if (threadIdx.x < NQ) {
-    const int qi = Q0 + (int)threadIdx.x;
-    const double v = sjab[threadIdx.x];
-    if (v != 0.0) unsafeAtomicAdd(&J[(size_t)(i0 + qi / C::NSB) * n + j0 + qi % C::NSB], v);
+  if (doJ) {
+    sfor<NQ>([&](auto qq) {
+      const double tot = wave_sum(jij[qq]);
+      if (lane == 0 && tot != 0.0) {
+        if (det)
+          fx_lds_add(sjab + 2 * qq, tot, bad);
+        else
+          atomicAdd(reinterpret_cast<double*>(sjab) + qq, tot);
+      }
+    });
   }
-  if (kscale != 0.0) {
+  if (bad) *ex.fx_over = 1;
+  __syncthreads();
+  if (doJ && threadIdx.x < NQ) {
+    const int qi = Q0 + (int)threadIdx.x;
+    const size_t e = (size_t)(i0 + qi / C::NSB) * n + j0 + qi % C::NSB;
+    if (det) {
+      fx_flush(sjab + 2 * threadIdx.x, ex.Jx + 2 * e);
+    } else {
+      const double v = reinterpret_cast<const double*>(sjab)[threadIdx.x];
+      if (v != 0.0) unsafeAtomicAdd(&J[e], v);
+    }
+  }
+  if (doK) {
     const double kq = 0.25 * kscale;
     for (int r = 0; r < S::NR; ++r) {
       const size_t ao = r < S::NMA ? (size_t)(i0 + S::MA0 + r) : (size_t)(j0 + S::MB_LO + (r - S::NMA));
-      for (int c = threadIdx.x; c < nbf; c += blockDim.x) {
-        const double v = Ks[(size_t)r * nbf + c];
-        if (v != 0.0) unsafeAtomicAdd(&K[ao * n + c], kq * v);
+      for (int c = threadIdx.x; c < W; c += blockDim.x) {
+        if (det) {
+          // the exchange scale is applied when the fixed-point sums are read
+          fx_flush(Kf + (size_t)r * krow + 2 * (size_t)c, ex.Kx + 2 * (ao * n + c0 + c));
+        } else {
+          const double v = Ks[(size_t)r * krow + c];
+          if (v != 0.0) unsafeAtomicAdd(&K[ao * n + c0 + c], kq * v);
+        }
       }
     }
   }
@@ -346,23 +412,49 @@ struct ClassOps {
   // all of them, so light classes use large blocks to keep enough waves per
   // CU resident (the register budget per thread shrinks with the block)
   static constexpr int SEG_BLK =
-      K::F ? 256 : ((C::L <= 2 && C::NBRA <= 5) ? 1024 : (C::L <= 3 ? 512 : 256));
+      K::F ? 256 : (C::L <= 1 ? 1024 : (C::L <= 3 ? 512 : 256));
   // doubles of the Schwarz staging block per pair (classes with f shells)
   static constexpr int SCHWARZ_BLOCK = K::F ? C::NBRA * C::NKETC : 0;
 
-  // dynamic LDS bytes of the segment kernel (K rows, and P rows with prow),
-  // plus its static J_ab slots when checked against the LDS limit
+  // dynamic LDS bytes of a segment block over W exchange columns: the K rows
+  // (doubles, or fixed-point pairs in deterministic builds) and the P rows
+  // with prow; seg_total adds the static J_ab slots
   template <int Q0, int Q1>
-  static size_t seg_bytes(int nbf, bool prow) {
-    return (size_t)SegRows<C, Q0, Q1>::NR * nbf * sizeof(double) * (prow ? 2 : 1);
+  static size_t seg_bytes(int W, bool prow, bool det) {
+    return (size_t)SegRows<C, Q0, Q1>::NR * W * sizeof(double) * ((det ? 2 : 1) + (prow ? 1 : 0));
   }
   template <int Q0, int Q1>
-  static size_t seg_total(int nbf, bool prow) {
-    return seg_bytes<Q0, Q1>(nbf, prow) + (size_t)(Q1 - Q0) * sizeof(double);
+  static size_t seg_total(int W, bool prow, bool det) {
+    return seg_bytes<Q0, Q1>(W, prow, det) + 2 * (size_t)(Q1 - Q0) * sizeof(double);
+  }
+  // the widest W whose block fits lim bytes
+  template <int Q0, int Q1>
+  static int seg_width(size_t lim, bool prow, bool det) {
+    const size_t fixed = 2 * (size_t)(Q1 - Q0) * sizeof(double);
+    const size_t per = seg_bytes<Q0, Q1>(1, prow, det);
+    return lim > fixed ? (int)std::min<size_t>((lim - fixed) / per, (size_t)1 << 30) : 0;
+  }
+
+  // shell-aligned column tiles of at most wmax columns covering [0, nbf)
+  static bool make_tiles(const JKLaunch& a, int wmax, SegExtra& ex) {
+    ex.ntile = 0;
+    int lo = 0;
+    while (lo < a.nbf) {
+      if (ex.ntile == kMaxColTiles || a.sh_bf0 == nullptr) return false;
+      // the last shell boundary within lo + wmax
+      const int* e = std::upper_bound(a.sh_bf0, a.sh_bf0 + a.nsh + 1, lo + wmax);
+      const int hi = *(e - 1);
+      if (hi <= lo) return false;  // one shell wider than a tile
+      ex.tile_lo[ex.ntile] = lo;
+      ex.tile_hi[ex.ntile] = hi;
+      ++ex.ntile;
+      lo = hi;
+    }
+    return true;
   }
 
   template <int Q0, int Q1, int K0, int K1, typename R, bool PROW>
-  static bool launch_seg(const JKLaunch& a, hipStream_t s, size_t bytes) {
+  static bool launch_seg(const JKLaunch& a, hipStream_t s, size_t bytes, const SegExtra& ex) {
     auto kern = k_eri_jk_seg<C, Q0, Q1, R, PROW, K0, K1, SEG_BLK>;
     if (bytes > 65536 &&
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
@@ -375,19 +467,16 @@ struct ClassOps {
     const long long per = a.ntask / std::max(a.nseg, 1);
     const long long want = (2048 + a.nseg - 1) / std::max(a.nseg, 1);
     const int nsplit = (int)std::max(1LL, std::min({want, per / (2 * SEG_BLK), 16LL}));
-    const dim3 grid((unsigned)((long long)a.nseg * nsplit));
+    const dim3 grid((unsigned)((long long)a.nseg * nsplit * ex.ntile));
     if constexpr (sizeof(R) == 4)
       hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos32, a.pk32,
-                         *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst, nsplit);
+                         *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst, nsplit, ex);
     else
       hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos, a.pk,
-                         *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst,
-                         nsplit);
+                         *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst, nsplit, ex);
     return true;
   }
 
-  // segment kernel with K and P rows in LDS when they fit, else K rows only
-  // (P from L2); a negative lds_max forces the K-only variant
   // waves of this kernel one CU can hold when a block needs `bytes` of LDS
   // (the CU's LDS equals the per-block maximum on gfx950; 32 wave slots)
   static int waves_per_cu(size_t bytes, size_t lds) {
@@ -396,21 +485,42 @@ struct ClassOps {
     return (int)std::min(blocks_lds, blocks_slots) * (SEG_BLK / 64);
   }
 
+  // The segment kernel with K and P rows in LDS when they fit (compiled when
+  // BOTH), else K rows only (P from L2; a negative lds_max forces it); when
+  // even whole K rows do not fit, the K rows are split into shell-aligned
+  // column tiles, each block evaluating the tasks whose ket shells fall in
+  // its tile.  kcol_tile > 0 caps the tile width (to exercise the tiled path
+  // on small systems).
   template <int Q0, int Q1, int K0, int K1, typename R, bool BOTH>
   static void seg(const JKLaunch& a, hipStream_t s) {
     static_assert(SegRows<C, Q0, Q1>::NR <= kSegRowsMax, "chunk needs more LDS rows than planned");
     if (a.nseg <= 0 || a.segs == nullptr) throw std::runtime_error("segment list missing");
     const size_t lim = (size_t)(a.lds_max < 0 ? -a.lds_max : a.lds_max);
-    const size_t with_p = seg_bytes<Q0, Q1>(a.nbf, true), k_only = seg_bytes<Q0, Q1>(a.nbf, false);
-    const size_t tot_p = seg_total<Q0, Q1>(a.nbf, true), tot_k = seg_total<Q0, Q1>(a.nbf, false);
-    // density rows in LDS unless that leaves fewer than seg_min_waves waves
-    // per CU and dropping them lets more reside
-    const int wp = waves_per_cu(tot_p, lim), wk = waves_per_cu(tot_k, lim);
-    const bool prow = a.lds_max > 0 && tot_p <= lim && (!BOTH || wp >= wk || wp >= a.seg_min_waves);
-    if (prow && launch_seg<Q0, Q1, K0, K1, R, true>(a, s, with_p)) return;
-    if constexpr (BOTH)
-      if (tot_k <= lim && launch_seg<Q0, Q1, K0, K1, R, false>(a, s, k_only)) return;
-    throw std::runtime_error("the exchange rows of one bra pair exceed the LDS of a block (nbf too large)");
+    const bool det = a.det != 0;
+    SegExtra ex{a.pmax, a.prim_screen, a.det, a.Jx, a.Kx, a.fx_over, 1, {0}, {a.nbf}};
+    const int cap = a.kcol_tile > 0 ? a.kcol_tile : a.nbf;
+    if (cap >= a.nbf) {
+      const size_t tot_k = seg_total<Q0, Q1>(a.nbf, false, det);
+      if constexpr (BOTH) {
+        // density rows in LDS unless that leaves fewer than seg_min_waves
+        // waves per CU and dropping them lets more reside
+        const size_t tot_p = seg_total<Q0, Q1>(a.nbf, true, det);
+        const int wp = waves_per_cu(tot_p, lim), wk = waves_per_cu(tot_k, lim);
+        const bool prow = a.lds_max > 0 && tot_p <= lim && (wp >= wk || wp >= a.seg_min_waves);
+        if (prow && launch_seg<Q0, Q1, K0, K1, R, true>(a, s, seg_bytes<Q0, Q1>(a.nbf, true, det), ex)) return;
+      }
+      if (tot_k <= lim && launch_seg<Q0, Q1, K0, K1, R, false>(a, s, seg_bytes<Q0, Q1>(a.nbf, false, det), ex))
+        return;
+    }
+    // tiles: the FP64 K-row kernel (FP32 tasks too)
+    const int w = std::min(cap, seg_width<Q0, Q1>(lim, false, det));
+    if (!make_tiles(a, w, ex))
+      throw std::runtime_error("the exchange rows of one bra pair need more LDS column tiles than a launch holds "
+                               "(nbf too large)");
+    int wmax = 0;
+    for (int t = 0; t < ex.ntile; ++t) wmax = std::max(wmax, ex.tile_hi[t] - ex.tile_lo[t]);
+    if (launch_seg<Q0, Q1, K0, K1, double, false>(a, s, seg_bytes<Q0, Q1>(wmax, false, det), ex)) return;
+    throw std::runtime_error("segment kernel launch configuration rejected");
   }
 
   template <int C0, int C1, bool LAST>
@@ -427,9 +537,10 @@ struct ClassOps {
     });
   }
 
-  // FP64 segment kernels only (FP32 tasks are evaluated in FP64): the variant
-  // compiled for classes with f shells, where the per-task kernel would double
-  // the compile time
+  // FP64 segment kernels with K rows only (FP32 tasks are evaluated in FP64):
+  // the variant compiled for classes with f shells, where more variants
+  // would multiply the compile time; their integral work outweighs reading
+  // the density from L2
   template <int C0, int C1, bool LAST>
   static void jk_seg_only_range(const JKLaunch& a, hipStream_t s) {
     static_assert(C0 < C1 && C1 <= NCH && (!LAST || C1 == NCH), "chunk range does not match the class");

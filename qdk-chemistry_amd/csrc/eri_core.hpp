@@ -83,6 +83,12 @@ struct RysArgsT {
   R hw[9][8];         //               w_i = hw/sqrt(T)
   const R* boys;      // Boys Taylor table [BOYS_NG][BOYS_W]: F_m(k * BOYS_DT)
   const R* h2;        // 1/(2p) per primitive pair (p = exponent sum), in ppos order
+  // primitive-quartet screening factor per primitive pair, in ppos order:
+  // sig = ln(|pk| p^-1/4) + ln(2^1/2 pi^5/2) / 2, so that the prefactor of a
+  // primitive quartet, 2 pi^5/2 |pk_ab pk_cd| / sqrt(p + q), never exceeds
+  // exp(sig_ab + sig_cd) (p + q >= 2 sqrt(pq)); each pair's primitives are
+  // sorted by sig, largest first
+  const R* sig;
 };
 using RysArgs = RysArgsT<double>;
 using RysArgs32 = RysArgsT<float>;
@@ -331,10 +337,29 @@ QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, 
 // Contract all primitive quartets of (bra|ket) into acc[q][k] for the bra
 // function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and the ket
 // cartesian pairs K0 + k, k < K1 - K0 (row-major over (c, d)), in precision R.
+// Primitive quartets whose prefactor bound exp(sig_ab + sig_cd) is below
+// exp(lnp) are skipped, as the reference's Libint2 engine skips those below
+// its precision (ScreeningMethod::Original at eps / max|P|,
+// libint2_direct.cpp:279-292); the primitives of a pair are sorted by sig,
+// so the kept ones form a prefix of both loops.  lnp = -inf keeps all.
+// ln of the primitive-quartet cut.  FP64: the reference's engine precision
+// max(eps / max|P|, eps / 1e10) (libint2_direct.cpp:279-287).  FP32 tasks
+// (Q_ab Q_cd below the mixed threshold, 1e-6) carry ~1e-13 absolute rounding
+// of their own; they cut at 1e-14 / max|P|, clamped the same way.
+template <typename R>
+QFE_DI R prim_ln_cut(const double* pmax, int prim_screen) {
+  if (!prim_screen) return R(-3.0e38);
+  const double pm = *pmax;
+  const double eps = 2.220446049250313e-16;
+  double prec = pm > 0.0 ? fmax(eps / pm, eps * 1e-10) : eps;
+  if (sizeof(R) == 4) prec = fmax(prec, pm > 0.0 ? fmax(1e-14 / pm, 1e-24) : 1e-14);
+  return (R)log(prec);
+}
+
 template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC>
 QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
                            const typename Vec4<R>::type* __restrict__ ppos, const R* __restrict__ pk,
-                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][K1 - K0]) {
+                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][K1 - K0], R lnp = R(-3.0e38)) {
   constexpr int NK = K1 - K0;
   const R TWO_PI_52 = R(34.98683665524972);  // 2 pi^(5/2)
   sfor<Q1 - Q0>([&](auto q) { sfor<NK>([&](auto k) { acc[q][k] = R(0); }); });
@@ -342,15 +367,21 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
   const R CDx = R(ket.A[0] - ket.B[0]), CDy = R(ket.A[1] - ket.B[1]), CDz = R(ket.A[2] - ket.B[2]);
   const R Ax = R(bra.A[0]), Ay = R(bra.A[1]), Az = R(bra.A[2]);
   const R Cx = R(ket.A[0]), Cy = R(ket.A[1]), Cz = R(ket.A[2]);
+  const R* __restrict__ sig = ra.sig;
+  const R sb0 = sig[bra.prim0];  // the bra's largest factor
   // ket primitives outer (per-lane data), bra primitives inner: the bra pair
   // is usually shared by the whole wave, so the inner loads are broadcasts
   for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
+    const R sk = sig[ik];
+    if (sk + sb0 < lnp) break;  // this and every later ket primitive
+    const R blim = lnp - sk;
     const auto Q = ppos[ik];
     const R Kk = pk[ik];
     const R q = Q.x;
     const R QCx = Q.y - Cx, QCy = Q.z - Cy, QCz = Q.w - Cz;
     const R h2q = ra.h2[ik];
     for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
+      if (sig[ib] < blim) break;  // this and every later bra primitive
       const auto P = ppos[ib];
       const R Kb = pk[ib];
       const R p = P.x;

@@ -67,41 +67,64 @@ QFE_DI void cart_density(const double* __restrict__ P, size_t n, int bx, int by,
   });
 }
 
+// exchange weights of one spin density: W[q][k] += wk (P_ac P_bd + P_ad P_bc)
+// over the cartesian chunk (bra cartesians [Q0, Q1), ket cartesians [K0, K1))
+template <class C, int Q0, int Q1, int K0, int K1>
+QFE_DI void exchange_weights(const double* __restrict__ P, size_t n, const PairHdr& bra, const PairHdr& ket,
+                             double wk, double (&W)[Q1 - Q0][K1 - K0]) {
+  double Pac[C::NCA][C::NCC], Pbd[C::NCB][C::NCD], Pad[C::NCA][C::NCD], Pbc[C::NCB][C::NCC];
+  cart_density<C::LA, C::LC, C::PURE>(P, n, bra.bfa, ket.bfa, Pac);
+  cart_density<C::LB, C::LD, C::PURE>(P, n, bra.bfb, ket.bfb, Pbd);
+  cart_density<C::LA, C::LD, C::PURE>(P, n, bra.bfa, ket.bfb, Pad);
+  cart_density<C::LB, C::LC, C::PURE>(P, n, bra.bfb, ket.bfa, Pbc);
+  sfor<Q1 - Q0>([&](auto qq) {
+    constexpr int ac = (Q0 + qq) / C::NCB, bc = (Q0 + qq) % C::NCB;
+    sfor<K1 - K0>([&](auto kk) {
+      constexpr int cc = (K0 + kk) / C::NCD, dc = (K0 + kk) % C::NCD;
+      W[qq][kk] = fma(wk, Pac[ac][cc] * Pbd[bc][dc] + Pad[ac][dc] * Pbc[bc][cc], W[qq][kk]);
+    });
+  });
+}
+
 // Derivatives of one canonical task (bra pair tk.x, ket pair tk.y) by its
 // centres A, B and C, contracted with the J and K weights: gj[centre][xyz],
-// gk[centre][xyz] (centre D follows by translation invariance).  Host-callable
-// so the arithmetic can be checked on the CPU.
+// gk[centre][xyz] (centre D follows by translation invariance).  P is the
+// density of the Coulomb term (the total density of a spin-split pair); the
+// exchange term sums over Pk (RHF: the total density, scale -1/4 of the
+// ordered sum) or over the spin densities Pk and Pk2 (scale -1/2), as
+// libint2_direct.cpp:683-692,866-886 do.  Host-callable so the arithmetic
+// can be checked on the CPU.
 template <class C, int Q0, int Q1, int K0, int K1>
 QFE_DI void grad_task(int2 tk, const PairHdr* __restrict__ pairs, const double4* __restrict__ ppos,
                       const double* __restrict__ pk, const double* __restrict__ pa, const RysArgs& ra,
-                      const double* __restrict__ P, int nbf, double kscale, double (&gj)[3][3],
-                      double (&gk)[3][3]) {
+                      const double* __restrict__ P, const double* __restrict__ Pk, const double* __restrict__ Pk2,
+                      int nbf, double kscale, double (&gj)[3][3], double (&gk)[3][3]) {
   using G = GradTraits<C>;
   const PairHdr bra = pairs[tk.x];
   const PairHdr ket = pairs[tk.y];
   const double deg = (bra.sa == bra.sb ? 1.0 : 2.0) * (ket.sa == ket.sb ? 1.0 : 2.0) *
                      (tk.x == tk.y ? 1.0 : 2.0);
   const size_t n = (size_t)nbf;
-  // cartesian density blocks
-  double Pab[C::NCA][C::NCB], Pcd[C::NCC][C::NCD], Pac[C::NCA][C::NCC], Pbd[C::NCB][C::NCD],
-      Pad[C::NCA][C::NCD], Pbc[C::NCB][C::NCC];
+  // cartesian density blocks of the Coulomb weight
+  double Pab[C::NCA][C::NCB], Pcd[C::NCC][C::NCD];
   cart_density<C::LA, C::LB, C::PURE>(P, n, bra.bfa, bra.bfb, Pab);
   cart_density<C::LC, C::LD, C::PURE>(P, n, ket.bfa, ket.bfb, Pcd);
-  cart_density<C::LA, C::LC, C::PURE>(P, n, bra.bfa, ket.bfa, Pac);
-  cart_density<C::LB, C::LD, C::PURE>(P, n, bra.bfb, ket.bfb, Pbd);
-  cart_density<C::LA, C::LD, C::PURE>(P, n, bra.bfa, ket.bfb, Pad);
-  cart_density<C::LB, C::LC, C::PURE>(P, n, bra.bfb, ket.bfa, Pbc);
   constexpr int NQ = Q1 - Q0, NK = K1 - K0;
   double WJ[NQ][NK], WK[NQ][NK];
-  const double wj = 0.5 * deg, wk = -0.125 * kscale * deg;
+  const double wj = 0.5 * deg;
+  // the canonical task stands for its deg ordered quartets: -1/4 (RHF) or
+  // -1/2 (per spin) of the ordered sum is -deg/8 or -deg/4 per task
+  const double wk = (Pk2 ? -0.25 : -0.125) * kscale * deg;
   sfor<NQ>([&](auto qq) {
     constexpr int ac = (Q0 + qq) / C::NCB, bc = (Q0 + qq) % C::NCB;
     sfor<NK>([&](auto kk) {
       constexpr int cc = (K0 + kk) / C::NCD, dc = (K0 + kk) % C::NCD;
       WJ[qq][kk] = wj * Pab[ac][bc] * Pcd[cc][dc];
-      WK[qq][kk] = wk * (Pac[ac][cc] * Pbd[bc][dc] + Pad[ac][dc] * Pbc[bc][cc]);
+      WK[qq][kk] = 0.0;
     });
   });
+  exchange_weights<C, Q0, Q1, K0, K1>(Pk, n, bra, ket, wk, WK);
+  if (Pk2) exchange_weights<C, Q0, Q1, K0, K1>(Pk2, n, bra, ket, wk, WK);
   sfor<3>([&](auto a) { sfor<3>([&](auto b) { gj[a][b] = 0.0; gk[a][b] = 0.0; }); });
   constexpr double TWO_PI_52 = 34.98683665524972;
   const double ABx = bra.A[0] - bra.B[0], ABy = bra.A[1] - bra.B[1], ABz = bra.A[2] - bra.B[2];
@@ -191,7 +214,8 @@ __global__ __launch_bounds__(256) void k_eri_grad(const int2* __restrict__ tasks
                                                   const PairHdr* __restrict__ pairs,
                                                   const double4* __restrict__ ppos, const double* __restrict__ pk,
                                                   const double* __restrict__ pa, RysArgs ra,
-                                                  const double* __restrict__ P, int nbf,
+                                                  const double* __restrict__ P, const double* __restrict__ Pk,
+                                                  const double* __restrict__ Pk2, int nbf,
                                                   const int* __restrict__ sh_atom, int natoms, double kscale,
                                                   int lds, double* __restrict__ gJ, double* __restrict__ gK) {
   extern __shared__ double sg[];
@@ -203,7 +227,7 @@ __global__ __launch_bounds__(256) void k_eri_grad(const int2* __restrict__ tasks
   if (t < ntask) {
     const int2 tk = tasks[t];
     double gj[3][3], gk[3][3];
-    grad_task<C, Q0, Q1, K0, K1>(tk, pairs, ppos, pk, pa, ra, P, nbf, kscale, gj, gk);
+    grad_task<C, Q0, Q1, K0, K1>(tk, pairs, ppos, pk, pa, ra, P, Pk, Pk2, nbf, kscale, gj, gk);
     const PairHdr& bra = pairs[tk.x];
     const PairHdr& ket = pairs[tk.y];
     const int at[4] = {sh_atom[bra.sa], sh_atom[bra.sb], sh_atom[ket.sa], sh_atom[ket.sb]};
@@ -251,7 +275,7 @@ struct GradOps {
       constexpr int q0 = b * K::BCH, q1 = (q0 + K::BCH < K::NBC) ? q0 + K::BCH : K::NBC;
       constexpr int k0 = k * K::KCH, k1 = (k0 + K::KCH < C::NKETC) ? k0 + K::KCH : C::NKETC;
       hipLaunchKernelGGL((k_eri_grad<C, q0, q1, k0, k1>), dim3((unsigned)grid), dim3(block), shm, s, a.tasks,
-                         a.ntask, a.pairs, a.ppos, a.pk, a.pa, *a.ra, a.P, a.nbf, a.sh_atom, a.natoms, a.kscale,
+                         a.ntask, a.pairs, a.ppos, a.pk, a.pa, *a.ra, a.P, a.Pk, a.Pk2, a.nbf, a.sh_atom, a.natoms, a.kscale,
                          lds, a.gJ, a.gK);
     });
   }

@@ -10,17 +10,18 @@
 #include <hipcub/hipcub.hpp>
 
 #include "eri_core.hpp"
+#include "fixed_point.hpp"
 #include "kernels.hpp"
 #include "screen.hpp"
 
 namespace qfe {
 
 // ------------------------------------------------------------ density norms
-// Pn is kept in float rounded towards +inf, so the screening test can only
-// keep more quartets than the double-precision test of the reference
+// Pn[s1][s2] = max |P_block(s1, s2)| in double, the norms the reference
+// screens with (compute_shellblock_norm, libint2_direct.cpp:43-59)
 __global__ void k_shell_norm(const double* __restrict__ P, int nbf, const int* __restrict__ sh_bf0,
-                             const int* __restrict__ sh_nf, int nsh, float* __restrict__ Pn,
-                             double* __restrict__ Pnd, int* __restrict__ bad) {
+                             const int* __restrict__ sh_nf, int nsh, double* __restrict__ Pn,
+                             int* __restrict__ bad) {
   const int s2 = blockIdx.x * blockDim.x + threadIdx.x;
   const int s1 = blockIdx.y;
   if (s2 >= nsh) return;
@@ -32,8 +33,7 @@ __global__ void k_shell_norm(const double* __restrict__ P, int nbf, const int* _
       if (!(v <= 1.0e300)) *bad = 1;  // NaN or Inf
       m = fmax(m, v);
     }
-  Pn[(size_t)s1 * nsh + s2] = __double2float_ru(m);
-  Pnd[(size_t)s1 * nsh + s2] = m;
+  Pn[(size_t)s1 * nsh + s2] = m;
 }
 
 __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __restrict__ out) {
@@ -67,14 +67,26 @@ __global__ void k_max_reduce(const double* __restrict__ x, size_t n, double* __r
 // of word k: position 64 k + j survived), so the J_cd reduction reads only
 // the slots that the class kernels wrote.
 //
+// Schwarz-prefix cut: thr / pmax lowered by 1e-12 relative, so that every
+// quartet it drops fails the exact density test whatever the rounding
+__device__ __forceinline__ double prefix_cut(double thr, double pmax) { return thr / pmax * (1.0 - 1.0e-12); }
+
 // One wave scans one item.  pn(c, d) returns max(Pn[a][c], Pn[a][d],
 // Pn[b][c], Pn[b][d]) for the bra shells (a, b); Pcd[q] = Pn of ket pair q.
+//
+// The density test is the reference's own, in double precision and with its
+// operand order: the reference's loop visits a quartet with the pair holding
+// the larger shell index (then the larger second index) as (s1, s2) and skips
+// it when max6 * Q(s1, s2) * Q(s3, s4) < threshold (libint2_direct.cpp:
+// 357-402), so the product is formed in that order here too.  The Schwarz
+// prefix cut uses thr / pmax lowered by 1e-12 relative: a quartet beyond the
+// prefix then fails the exact test in either product order.
 template <bool WRITE, class PN>
 __device__ __forceinline__ void scan_item(long long item, int lane, const int2* __restrict__ items, long long nitems,
                                           const int2* __restrict__ buckets, const int* __restrict__ pair_bucket,
                                           const int2* __restrict__ psh, const double* __restrict__ Q,
-                                          const float* __restrict__ Pcd, float pab, const PN& pn, double cut,
-                                          double thr, double mix_thr, int* __restrict__ counts,
+                                          const double* __restrict__ Pcd, double pab, int2 ab, const PN& pn,
+                                          double cut, double thr, double mix_thr, int* __restrict__ counts,
                                           const long long* __restrict__ offsets, int2* __restrict__ tasks,
                                           int nk, int* __restrict__ nslot, int* __restrict__ nmask,
                                           const long long* __restrict__ mbase,
@@ -87,6 +99,8 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
   int end = bk.y;
   if (pair_bucket[p] == it.y) end = p + 1;  // same bucket: q <= p
   const double Qp = Q[p];
+  // the reference's loop order of the bra pair: (larger shell, smaller shell)
+  const int pa1 = ab.x > ab.y ? ab.x : ab.y, pa2 = ab.x > ab.y ? ab.y : ab.x;
   const long long base64 = WRITE ? offsets[item] : 0;
   const long long base32 = WRITE ? offsets[nitems + item] : 0;
   unsigned long long* const mw = masks ? jmask + mbase[item] : nullptr;
@@ -103,8 +117,11 @@ __device__ __forceinline__ void scan_item(long long item, int lane, const int2* 
       pre = !(Qp * Qq < cut);
       if (pre) {
         const int2 cd = psh[q];
-        const float m = fmaxf(fmaxf(pab, Pcd[q]), pn(cd.x, cd.y));
-        ok = !((double)m * Qp * Qq < thr);  // libint2_direct.cpp:400-402
+        const double m = fmax(fmax(pab, Pcd[q]), pn(cd.x, cd.y));
+        const int qc1 = cd.x > cd.y ? cd.x : cd.y, qc2 = cd.x > cd.y ? cd.y : cd.x;
+        const bool bra_first = pa1 > qc1 || (pa1 == qc1 && pa2 >= qc2);
+        const double Q12 = bra_first ? Qp : Qq, Q34 = bra_first ? Qq : Qp;
+        ok = !(m * Q12 * Q34 < thr);  // libint2_direct.cpp:400-402
       }
       lo = Qp * Qq < mix_thr;
     }
@@ -138,7 +155,7 @@ __global__ __launch_bounds__(256) void k_task_scan_seg(const int2* __restrict__ 
                                                        long long nitems, const int2* __restrict__ buckets,
                                                        const int* __restrict__ pair_bucket,
                                                        const int2* __restrict__ psh, const double* __restrict__ Q,
-                                                       const float* __restrict__ Pn, const float* __restrict__ Pcd,
+                                                       const double* __restrict__ Pn, const double* __restrict__ Pcd,
                                                        int nsh, const double* __restrict__ pmax_ptr, double thr,
                                                        double mix_thr, int* __restrict__ counts,
                                                        const long long* __restrict__ offsets,
@@ -146,7 +163,7 @@ __global__ __launch_bounds__(256) void k_task_scan_seg(const int2* __restrict__ 
                                                        const int* __restrict__ cp_nk, int* __restrict__ nslot,
                                                        int* __restrict__ nmask, const long long* __restrict__ mbase,
                                                        unsigned long long* __restrict__ jmask) {
-  extern __shared__ float prow[];  // [2][nsh]: rows a and b of Pn
+  extern __shared__ double prow[];  // [2][nsh]: rows a and b of Pn
   const int2 sg = segs[blockIdx.x];
   const int2 ab = psh[items[sg.x].x];
   for (int c = threadIdx.x; c < nsh; c += blockDim.x) {
@@ -154,13 +171,13 @@ __global__ __launch_bounds__(256) void k_task_scan_seg(const int2* __restrict__ 
     prow[nsh + c] = Pn[(size_t)ab.y * nsh + c];
   }
   __syncthreads();
-  const float pab = prow[ab.y];
-  const double cut = thr / (*pmax_ptr);
+  const double pab = prow[ab.y];
+  const double cut = prefix_cut(thr, *pmax_ptr);
   const int nk = cp_nk[item_cp[sg.x]];
   const int lane = (int)(threadIdx.x & 63);
-  auto pn = [&](int c, int d) { return fmaxf(fmaxf(prow[c], prow[d]), fmaxf(prow[nsh + c], prow[nsh + d])); };
+  auto pn = [&](int c, int d) { return fmax(fmax(prow[c], prow[d]), fmax(prow[nsh + c], prow[nsh + d])); };
   for (int item = sg.x + (int)(threadIdx.x >> 6); item < sg.y; item += (int)(blockDim.x >> 6))
-    scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, pab, pn, cut, thr, mix_thr,
+    scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, pab, ab, pn, cut, thr, mix_thr,
                      counts, offsets, tasks, nk, nslot, nmask, mbase, jmask);
 }
 
@@ -171,7 +188,7 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
                                                    const int2* __restrict__ buckets,
                                                    const int* __restrict__ pair_bucket,
                                                    const int2* __restrict__ psh, const double* __restrict__ Q,
-                                                   const float* __restrict__ Pn, const float* __restrict__ Pcd,
+                                                   const double* __restrict__ Pn, const double* __restrict__ Pcd,
                                                    int nsh, const double* __restrict__ pmax_ptr, double thr,
                                                    double mix_thr, int* __restrict__ counts,
                                                    const long long* __restrict__ offsets,
@@ -183,17 +200,17 @@ __global__ __launch_bounds__(256) void k_task_scan(const int2* __restrict__ item
   const int lane = threadIdx.x & 63;
   if (item >= nitems) return;
   const int2 ab = psh[items[item].x];
-  const float* ra = Pn + (size_t)ab.x * nsh;
-  const float* rb = Pn + (size_t)ab.y * nsh;
-  auto pn = [&](int c, int d) { return fmaxf(fmaxf(ra[c], ra[d]), fmaxf(rb[c], rb[d])); };
-  scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, ra[ab.y], pn,
-                   thr / (*pmax_ptr), thr, mix_thr, counts, offsets, tasks, cp_nk[item_cp[item]], nslot, nmask, mbase,
-                   jmask);
+  const double* ra = Pn + (size_t)ab.x * nsh;
+  const double* rb = Pn + (size_t)ab.y * nsh;
+  auto pn = [&](int c, int d) { return fmax(fmax(ra[c], ra[d]), fmax(rb[c], rb[d])); };
+  scan_item<WRITE>(item, lane, items, nitems, buckets, pair_bucket, psh, Q, Pcd, ra[ab.y], ab, pn,
+                   prefix_cut(thr, *pmax_ptr), thr, mix_thr, counts, offsets, tasks, cp_nk[item_cp[item]], nslot,
+                   nmask, mbase, jmask);
 }
 
 // Pcd[q] = Pn of the shell pair of pair q
-__global__ void k_pair_norm(const int2* __restrict__ psh, int npair, const float* __restrict__ Pn, int nsh,
-                            float* __restrict__ Pcd) {
+__global__ void k_pair_norm(const int2* __restrict__ psh, int npair, const double* __restrict__ Pn, int nsh,
+                            double* __restrict__ Pcd) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < npair) Pcd[q] = Pn[(size_t)psh[q].x * nsh + psh[q].y];
 }
@@ -254,19 +271,42 @@ __global__ void k_symmetrize(double* __restrict__ J, double* __restrict__ K, int
   }
 }
 
+// J and K of a deterministic build from their fixed-point sums, symmetrised
+// like k_symmetrize: J = (J + J^T) / 4, K = kq (K + K^T) / 2 (kq, the
+// exchange scale (alpha + beta) / 4, is applied here rather than per update)
+__global__ void k_fixed_finish(const unsigned long long* __restrict__ Jx, const unsigned long long* __restrict__ Kx,
+                               double* __restrict__ J, double* __restrict__ K, int n, double kq) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j > i || i >= n) return;
+  const size_t a = (size_t)i * n + j, b = (size_t)j * n + i;
+  if (J) {
+    const double v = 0.25 * (fx_value((long long)Jx[2 * a], (long long)Jx[2 * a + 1]) +
+                             fx_value((long long)Jx[2 * b], (long long)Jx[2 * b + 1]));
+    J[a] = v;
+    J[b] = v;
+  }
+  if (K) {
+    const double v = 0.5 * kq * (fx_value((long long)Kx[2 * a], (long long)Kx[2 * a + 1]) +
+                                 fx_value((long long)Kx[2 * b], (long long)Kx[2 * b + 1]));
+    K[a] = v;
+    K[b] = v;
+  }
+}
+
 // ------------------------------------------------------------ host wrappers
-void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh,
-                       float* Pn, double* Pnd, int* bad, double* pmax, hipStream_t s) {
+void launch_shell_norm(const double* P, int nbf, const int* sh_bf0, const int* sh_nf, int nsh, double* Pn, int* bad,
+                       double* pmax, hipStream_t s) {
   dim3 block(64), grid((nsh + 63) / 64, nsh);
-  hipLaunchKernelGGL(k_shell_norm, grid, block, 0, s, P, nbf, sh_bf0, sh_nf, nsh, Pn, Pnd, bad);
+  hipLaunchKernelGGL(k_shell_norm, grid, block, 0, s, P, nbf, sh_bf0, sh_nf, nsh, Pn, bad);
   (void)hipMemsetAsync(pmax, 0, sizeof(double), s);
   size_t n = (size_t)nsh * nsh;
   int g = (int)((n + 255) / 256);
   if (g > 1024) g = 1024;
-  hipLaunchKernelGGL(k_max_reduce, dim3(g), dim3(256), 0, s, Pnd, n, pmax);
+  hipLaunchKernelGGL(k_max_reduce, dim3(g), dim3(256), 0, s, Pn, n, pmax);
 }
 
-void launch_pair_norm(const int2* psh, int npair, const float* Pn, int nsh, float* Pcd, hipStream_t s) {
+void launch_pair_norm(const int2* psh, int npair, const double* Pn, int nsh, double* Pcd, hipStream_t s) {
   if (npair <= 0) return;
   hipLaunchKernelGGL(k_pair_norm, dim3((npair + 255) / 256), dim3(256), 0, s, psh, npair, Pn, nsh, Pcd);
 }
@@ -275,7 +315,7 @@ template <bool WRITE>
 static void task_scan(const ScanArgs& a, int* counts, const long long* offsets, int2* tasks, int* nslot,
                       int* nmask, const long long* mbase, unsigned long long* jmask, hipStream_t s) {
   if (a.nitems <= 0) return;
-  const size_t lds = 2 * sizeof(float) * (size_t)a.nsh;
+  const size_t lds = 2 * sizeof(double) * (size_t)a.nsh;
   if (a.segs && a.nseg > 0 && lds <= (size_t)a.lds_max) {
     auto kern = k_task_scan_seg<WRITE>;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -342,6 +382,13 @@ void launch_symmetrize(double* J, double* K, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_symmetrize, grid, block, 0, s, J, K, n);
 }
 
+void launch_fixed_finish(const unsigned long long* Jx, const unsigned long long* Kx, double* J, double* K, int n,
+                         double kq, hipStream_t s) {
+  if (!J && !K) return;
+  dim3 block(256), grid((n + 255) / 256, n);
+  hipLaunchKernelGGL(k_fixed_finish, grid, block, 0, s, Jx, Kx, J, K, n, kq);
+}
+
 // ------------------------------------------------------------ J_cd by ket pair
 // Share slots: item (p, kb) owns nslot[item] / NK positions j (ket pair
 // q = first pair of bucket kb + j), NK doubles each, from jbuf + jslot[item].
@@ -362,7 +409,8 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
                                                    const unsigned long long* __restrict__ jmask,
                                                    const double* __restrict__ jbuf,
                                                    const PairHdr* __restrict__ pairs, int nsb,
-                                                   double* __restrict__ J, int nbf) {
+                                                   double* __restrict__ J, int nbf,
+                                                   unsigned long long* __restrict__ Jx, int* __restrict__ over) {
   const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int lane = (int)(threadIdx.x & 63);
   if (w >= ntiles) return;  // whole waves
@@ -444,41 +492,48 @@ __global__ __launch_bounds__(256) void k_jcd_tiles(const int4* __restrict__ tile
         }
       }
     }
+    int bad = 0;
 #pragma unroll
     for (int f = 0; f < NK; ++f) {
       const int e = f * 64 + lane, j = jc + e / NK, fn = e % NK;
       if (j < nmax && acc[f] != 0.0) {
         const PairHdr* h = pairs + tl.z + j;
-        unsafeAtomicAdd(&J[(size_t)(h->bfa + fn / nsb) * nbf + h->bfb + fn % nsb], acc[f]);
+        const size_t e = (size_t)(h->bfa + fn / nsb) * nbf + h->bfb + fn % nsb;
+        if (Jx)
+          fx_global_add(Jx + 2 * e, acc[f], bad);
+        else
+          unsafeAtomicAdd(&J[e], acc[f]);
       }
     }
+    if (bad) *over = 1;
   }
 }
 
 template <int NK>
 static void jcd_tiles_launch(const int4* tiles, int ntiles, const int* nslot, const long long* jslot,
                              const long long* mbase, const unsigned long long* jmask, const double* jbuf,
-                             const PairHdr* pairs, int nsb, double* J, int nbf, hipStream_t s) {
+                             const PairHdr* pairs, int nsb, double* J, int nbf, unsigned long long* Jx, int* over,
+                             hipStream_t s) {
   const long long threads = (long long)ntiles * 64;
   hipLaunchKernelGGL((k_jcd_tiles<NK>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tiles, ntiles,
-                     nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf);
+                     nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over);
 }
 
 void launch_jcd_tiles(const int4* tiles, int ntiles, int nk, int nsb, const int* nslot, const long long* jslot,
                       const long long* mbase, const unsigned long long* jmask, const double* jbuf,
-                      const PairHdr* pairs, double* J, int nbf, hipStream_t s) {
+                      const PairHdr* pairs, double* J, int nbf, unsigned long long* Jx, int* over, hipStream_t s) {
   if (ntiles <= 0) return;
   switch (nk) {
-    case 1: jcd_tiles_launch<1>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 3: jcd_tiles_launch<3>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 5: jcd_tiles_launch<5>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 7: jcd_tiles_launch<7>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 9: jcd_tiles_launch<9>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 15: jcd_tiles_launch<15>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 21: jcd_tiles_launch<21>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 25: jcd_tiles_launch<25>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 35: jcd_tiles_launch<35>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
-    case 49: jcd_tiles_launch<49>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, s); break;
+    case 1: jcd_tiles_launch<1>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 3: jcd_tiles_launch<3>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 5: jcd_tiles_launch<5>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 7: jcd_tiles_launch<7>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 9: jcd_tiles_launch<9>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 15: jcd_tiles_launch<15>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 21: jcd_tiles_launch<21>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 25: jcd_tiles_launch<25>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 35: jcd_tiles_launch<35>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
+    case 49: jcd_tiles_launch<49>(tiles, ntiles, nslot, jslot, mbase, jmask, jbuf, pairs, nsb, J, nbf, Jx, over, s); break;
     default: break;  // no other pure pair sizes up to f
   }
 }
@@ -494,6 +549,48 @@ void launch_copy_words(const unsigned long long* src, unsigned long long* dst, s
   const size_t want = (n + 255) / 256;
   const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
   hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(256), 0, s, src, dst, n);
+}
+
+
+// ------------------------------------------------------------ profiling
+template <typename R>
+__global__ __launch_bounds__(256) void k_count_prims(const int2* __restrict__ tasks, long long ntask,
+                                                     const PairHdr* __restrict__ pairs, const R* __restrict__ sig,
+                                                     const double* __restrict__ pmax,
+                                                     int prim_screen, unsigned long long* __restrict__ count) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n = 0;
+  if (t < ntask) {
+    const int2 tk = tasks[t];
+    const PairHdr bra = pairs[tk.x], ket = pairs[tk.y];
+    const R lnp = prim_ln_cut<R>(pmax, prim_screen);
+    const R sb0 = sig[bra.prim0];
+    for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
+      const R sk = sig[ik];
+      if (sk + sb0 < lnp) break;
+      const R blim = lnp - sk;
+      for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
+        if (sig[ib] < blim) break;
+        ++n;
+      }
+    }
+  }
+  // one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(count, n);
+}
+
+void launch_count_prims(const int2* tasks, long long ntask, const PairHdr* pairs, const double* sig,
+                        const float* sig32, const double* pmax, int prim_screen, int fp32,
+                        unsigned long long* count, hipStream_t s) {
+  if (ntask <= 0) return;
+  const dim3 grid((unsigned)((ntask + 255) / 256));
+  if (fp32)
+    hipLaunchKernelGGL(k_count_prims<float>, grid, dim3(256), 0, s, tasks, ntask, pairs, sig32, pmax,
+                       prim_screen, count);
+  else
+    hipLaunchKernelGGL(k_count_prims<double>, grid, dim3(256), 0, s, tasks, ntask, pairs, sig, pmax,
+                       prim_screen, count);
 }
 
 }  // namespace qfe

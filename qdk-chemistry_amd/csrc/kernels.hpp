@@ -16,6 +16,19 @@ struct RysArgsT;
 using RysArgs = RysArgsT<double>;
 using RysArgs32 = RysArgsT<float>;
 
+// Per-launch extras of the segment J/K kernel (passed by value).
+constexpr int kMaxColTiles = 8;
+struct SegExtra {
+  const double* pmax;       // max shell-block norm of the screening density (device scalar)
+  int prim_screen;          // 1: skip primitive quartets below the engine precision
+  int det;                  // 1: order-independent fixed-point accumulation (Jx, Kx)
+  unsigned long long* Jx;   // (hi, lo) pairs per J element, deterministic builds
+  unsigned long long* Kx;   // (hi, lo) pairs per K element, deterministic builds
+  int* fx_over;             // set to 1 if a value leaves the fixed-point range
+  int ntile;                // column tiles of the exchange rows (1: whole rows)
+  int tile_lo[kMaxColTiles], tile_hi[kMaxColTiles];  // [lo, hi) columns of each tile (shell-aligned)
+};
+
 struct JKLaunch {
   const int2* tasks;
   long long ntask;
@@ -50,6 +63,16 @@ struct JKLaunch {
   // keep the density rows in LDS when that still leaves at least this many
   // resident waves per CU (else the K-row-only variant, if it fits more)
   int seg_min_waves;
+  // primitive screening, deterministic accumulation and column tiles
+  const double* pmax;
+  int prim_screen;
+  int det;
+  unsigned long long* Jx;
+  unsigned long long* Kx;
+  int* fx_over;
+  const int* sh_bf0;        // host: first function of each shell, nsh + 1 entries (tile boundaries)
+  int nsh;
+  int kcol_tile;            // > 0: exchange-row tiles of at most this many columns (testing knob)
 };
 
 struct GradLaunch {
@@ -60,13 +83,16 @@ struct GradLaunch {
   const double* pk;
   const double* pa;       // exponent of each primitive pair's first shell
   const RysArgs* ra;
-  const double* P;
+  const double* P;        // Coulomb density (the total density of a spin pair)
+  const double* Pk;       // exchange density: P itself (RHF) or the alpha density
+  const double* Pk2;      // null (RHF) or the beta density
   int nbf;
   const int* sh_atom;
   int natoms;
   double kscale;
   double* gJ;             // 3 * natoms, xyz-major
   double* gK;
+  const double* sph;      // device copy of the solid-harmonic table kSph (runtime-l kernel)
 };
 
 struct SchwarzLaunch {
@@ -87,4 +113,9 @@ int schwarz_block(int pair_cls);
 void launch_jk(int bra_cls, int ket_cls, bool pure, const JKLaunch& a, hipStream_t s);
 void launch_schwarz(int pair_cls, bool pure, const SchwarzLaunch& a, hipStream_t s);
 void launch_grad(int bra_cls, int ket_cls, bool pure, const GradLaunch& a, hipStream_t s);
+// highest shell angular momentum the gradient kernels of this build cover
+int grad_max_l();
+// the gradient of one class pair with runtime angular momenta (classes with
+// f shells; eri_grad_any.hip)
+void launch_grad_any(const GradLaunch& a, int la, int lb, int lc, int ld, hipStream_t s);
 }  // namespace qfe

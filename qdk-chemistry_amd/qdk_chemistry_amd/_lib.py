@@ -28,7 +28,8 @@ class QfeBasis(ctypes.Structure):
 class QfeConfig(ctypes.Structure):
     _fields_ = [("eri_threshold", ctypes.c_double), ("shell_pair_threshold", ctypes.c_double),
                 ("device_id", ctypes.c_int), ("mix_precision", ctypes.c_int),
-                ("mix_threshold", ctypes.c_double), ("verbose", ctypes.c_int), ("profile", ctypes.c_int)]
+                ("mix_threshold", ctypes.c_double), ("verbose", ctypes.c_int), ("profile", ctypes.c_int),
+                ("use_atomics", ctypes.c_int)]
 
 
 class QfeStats(ctypes.Structure):
@@ -42,7 +43,7 @@ EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients",
            "qfe_one_body_grad", "qfe_one_body_gpu", "qfe_one_body_grad_gpu", "qfe_eigh_gpu",
            "qfe_dfj_create", "qfe_dfj_build_j", "qfe_dfj_naux", "qfe_dfj_destroy",
            "qfe_quarter_trans", "qfe_mo_eri", "qfe_scf_create", "qfe_scf_set_density",
-           "qfe_scf_step", "qfe_scf_update", "qfe_scf_finish", "qfe_scf_get", "qfe_scf_destroy", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_nbf", "qfe_max_l", "qfe_destroy",
+           "qfe_scf_step", "qfe_scf_update", "qfe_scf_finish", "qfe_scf_get", "qfe_scf_destroy", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_class_chunks", "qfe_nbf", "qfe_max_l", "qfe_destroy",
            "qfe_last_error"]
 
 _lib = None
@@ -68,7 +69,7 @@ def lib():
         L.qfe_build_jk_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
         L.qfe_gradients.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_double]
+                                    ctypes.c_double, ctypes.c_int]
         L.qfe_one_body.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp]
         L.qfe_one_body_grad.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, _dp]
         L.qfe_one_body_gpu.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp, ctypes.c_int]
@@ -96,7 +97,8 @@ def lib():
         L.qfe_device_buffers.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                          ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
         L.qfe_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_int]
-        L.qfe_class_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp]
+        L.qfe_class_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp]
+        L.qfe_class_chunks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip]
         L.qfe_nbf.argtypes = [ctypes.c_void_p]
         L.qfe_destroy.argtypes = [ctypes.c_void_p]
         L.qfe_destroy.restype = None

@@ -35,7 +35,9 @@ class ERIConfig:
     method: ERIMethod = ERIMethod.Rys
     eri_threshold: float = 1e-12
     shell_pair_threshold: float = 1e-12
-    use_atomics: bool = True
+    # False (the reference's default, core/scf.h:81-82): bitwise-repeatable
+    # J/K through order-independent fixed-point sums; True: FP64 atomics
+    use_atomics: bool = False
     is_mix_precision: bool = False
     mix_precision_threshold: float = 1e-6
     device_id: int = 0
@@ -54,7 +56,8 @@ class RysERI:
         self.nbf = basis.nbf
         self._b = _lib.BasisArrays(basis, structure)
         c = _lib.QfeConfig(cfg.eri_threshold, cfg.shell_pair_threshold, cfg.device_id,
-                           int(cfg.is_mix_precision), cfg.mix_precision_threshold, cfg.verbose, int(cfg.profile))
+                           int(cfg.is_mix_precision), cfg.mix_precision_threshold, cfg.verbose, int(cfg.profile),
+                           int(cfg.use_atomics))
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().qfe_create(ctypes.byref(self._b.c), ctypes.byref(c), ctypes.byref(h)))
         self.h = h
@@ -71,14 +74,19 @@ class RysERI:
         except Exception:
             pass
 
-    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0):
-        """(J, K) for a host density (ndm x nbf x nbf or nbf x nbf)."""
+    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0, J=True, K=True):
+        """(J, K) for a host density (ndm x nbf x nbf or nbf x nbf).  J=False
+        or K=False skips that matrix (None is returned in its place), as the
+        reference's build_JK does for a null output (eri_multiplexer.cpp:66-67)."""
         P = np.ascontiguousarray(P, dtype=np.float64)
         ndm = P.size // (self.nbf * self.nbf)
-        J = np.empty_like(P)
-        K = np.empty_like(P)
-        _lib.check(_lib.lib().qfe_build_jk(self.h, _lib.ptr(P), _lib.ptr(J), _lib.ptr(K), alpha, beta, omega, ndm))
-        return J, K
+        if ndm * self.nbf * self.nbf != P.size or ndm not in (1, 2):
+            raise ValueError("build_JK takes 1 or 2 densities of nbf x nbf")
+        Jo = np.empty_like(P) if J else None
+        Ko = np.empty_like(P) if K else None
+        _lib.check(_lib.lib().qfe_build_jk(self.h, _lib.ptr(P), _lib.ptr(Jo) if J else None,
+                                           _lib.ptr(Ko) if K else None, alpha, beta, omega, ndm))
+        return Jo, Ko
 
     def device_buffers(self):
         """Engine-owned HBM buffers (P, J, K) as raw device addresses."""
@@ -105,14 +113,22 @@ class RysERI:
                                                   ctypes.c_void_p(dK) if dK is not None else None,
                                                   alpha, beta, omega, s))
 
-    def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0):
-        """(dJ, dK): 3*natoms each, xyz-major (libint2_direct.cpp:824)."""
+    def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0, dJ=True, dK=True):
+        """(dJ, dK): 3*natoms each, xyz-major (libint2_direct.cpp:824).  P is
+        the RHF total density or the spin pair [P_alpha, P_beta] (Coulomb term
+        of P_alpha + P_beta, exchange summed over the spins with scale -1/2,
+        :683-692,866-886).  dJ=False or dK=False skips that term (None in its
+        place), as a null output does in the reference."""
         P = np.ascontiguousarray(P, dtype=np.float64)
+        ndm = P.size // (self.nbf * self.nbf)
+        if ndm * self.nbf * self.nbf != P.size or ndm not in (1, 2):
+            raise ValueError("get_gradients takes 1 or 2 densities of nbf x nbf")
         na = self.structure.natoms
-        dJ = np.zeros(3 * na)
-        dK = np.zeros(3 * na)
-        _lib.check(_lib.lib().qfe_gradients(self.h, _lib.ptr(P), _lib.ptr(dJ), _lib.ptr(dK), alpha, beta, omega))
-        return dJ, dK
+        gJ = np.zeros(3 * na) if dJ else None
+        gK = np.zeros(3 * na) if dK else None
+        _lib.check(_lib.lib().qfe_gradients(self.h, _lib.ptr(P), _lib.ptr(gJ) if dJ else None,
+                                            _lib.ptr(gK) if dK else None, alpha, beta, omega, ndm))
+        return gJ, gK
 
     def quarter_trans(self, C):
         """out[i, j, k, p] = sum_l (ij|kl) C[l, p] for C (nbf x nt): the first
@@ -133,15 +149,19 @@ class RysERI:
         _lib.check(_lib.lib().qfe_set_precision(self.h, int(mix), thr))
         self.mixed = bool(mix)
 
-    def class_stats(self, precision=0):
+    def class_stats(self, precision=0, executed=False):
         """[((la, lb, lc, ld), quartets, prim_quartets, ms)] of the last build,
-        for the FP64 (0) or FP32 (1) tasks."""
+        for the FP64 (0) or FP32 (1) tasks; prim_quartets is the census over
+        significant primitive pairs, or with executed=True (profiled builds)
+        the primitive quartets the kernels evaluated after primitive
+        screening."""
         cap = 256
         cls = np.zeros(4 * cap, dtype=np.int32)
-        q, pq, ms = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        q, pq, epq, ms = np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap)
         n = _lib.lib().qfe_class_stats(self.h, precision, cap, _lib.ptr(cls, ctypes.c_int), _lib.ptr(q),
-                                       _lib.ptr(pq), _lib.ptr(ms))
-        return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), float(q[i]), float(pq[i]), float(ms[i]))
+                                       _lib.ptr(pq), _lib.ptr(epq), _lib.ptr(ms))
+        p = epq if executed else pq
+        return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), float(q[i]), float(p[i]), float(ms[i]))
                 for i in range(min(n, cap))]
 
     def stats(self) -> dict:
@@ -178,13 +198,29 @@ class DFJ:
         except Exception:
             pass
 
-    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0):
+    def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0, J=True, K=True):
+        """J of the fitted density; K is not fitted (None when K=False, else
+        zeros, as the reference's DF engine leaves it)."""
         P = np.ascontiguousarray(P, dtype=np.float64)
         if P.shape != (self.nbf, self.nbf):
             raise ValueError("DFJ.build_JK takes one nbf x nbf density")
-        J = np.empty_like(P)
-        _lib.check(_lib.lib().qfe_dfj_build_j(self.h, _lib.ptr(P), _lib.ptr(J)))
-        return J, np.zeros_like(P)
+        Jo = None
+        if J:
+            Jo = np.empty_like(P)
+            _lib.check(_lib.lib().qfe_dfj_build_j(self.h, _lib.ptr(P), _lib.ptr(Jo)))
+        return Jo, (np.zeros_like(P) if K else None)
+
+    def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0, dJ=True, dK=True):
+        """The reference's DF engine gives DF-J gradients only and refuses an
+        exchange gradient (ERI_DF::get_gradients, incore_impl_df.cpp:203-214);
+        the DF-J derivative integrals are not built on the GPU here."""
+        if abs(omega) > 1e-12:
+            raise RuntimeError("ERIINCORE_DF gradients Does Not Support Range-Separated Hybrids")
+        if dK and abs(alpha) + abs(beta) > 1e-12:
+            raise RuntimeError("ERIINCORE_DF + Exchange gradients is Not Yet Implemented")
+        if not dJ:
+            raise RuntimeError("ERIINCORE_DF is only valid for DF-J gradients")
+        raise NotImplementedError("DF-J nuclear gradients are not built in the MI355X engine")
 
 
 class MOERI:
@@ -209,7 +245,9 @@ class MOERI:
 class ERIMultiplexer:
     """Routes J and K to one engine, or J and K to two engines (J with omega 0),
     and gradients to a third when given; restates ERIMultiplexer
-    (eri_multiplexer.cpp:34-71): a single engine is called once for both."""
+    (eri_multiplexer.cpp:10-94): a single engine is called once for both, two
+    engines each build only their own matrix (the other output null), and
+    gradients follow the same split unless a gradient engine was given."""
 
     def __init__(self, j_engine, k_engine=None, grad_engine=None):
         self.j_impl = j_engine
@@ -219,12 +257,18 @@ class ERIMultiplexer:
     def build_JK(self, P, alpha=1.0, beta=0.0, omega=0.0):
         if self.j_impl is self.k_impl:
             return self.j_impl.build_JK(P, alpha, beta, omega)
-        J, _ = self.j_impl.build_JK(P, alpha, beta, 0.0)
-        _, K = self.k_impl.build_JK(P, alpha, beta, omega)
+        J, _ = self.j_impl.build_JK(P, alpha, beta, 0.0, K=False)
+        _, K = self.k_impl.build_JK(P, alpha, beta, omega, J=False)
         return J, K
 
     def get_gradients(self, P, alpha=1.0, beta=0.0, omega=0.0):
-        return self.grad_impl.get_gradients(P, alpha, beta, omega)
+        if self.grad_impl is not self.j_impl:
+            return self.grad_impl.get_gradients(P, alpha, beta, omega)
+        if self.j_impl is self.k_impl:
+            return self.j_impl.get_gradients(P, alpha, beta, omega)
+        dJ, _ = self.j_impl.get_gradients(P, alpha, beta, 0.0, dK=False)
+        _, dK = self.k_impl.get_gradients(P, alpha, beta, omega, dJ=False)
+        return dJ, dK
 
 
 def compiled_max_l() -> int:

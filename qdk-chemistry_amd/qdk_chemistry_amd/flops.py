@@ -1,21 +1,34 @@
-"""Algorithmic FP64 operation counts of the ERI class kernels.
-
-The count follows the source of csrc/eri_core.hpp / csrc/eri_class.hpp
-operation by operation (FMA = 2, add/mul/rcp/rsq = 1) and is the figure the
-roofline uses:
+"""Algorithmic operation counts of the ERI class kernels (the roofline's F).
 
     flops(class) = N_prim_quartets * f_prim(class) + N_quartets * f_contr(class)
 
-f_prim: one pass of the inner (bra primitive) loop body: Gaussian-product
-setup, roots and weights on their cheapest path (large-T limit for 3+ roots,
-large-T Boys moments for 1-2 roots), per root the 2-D recursion and the 1-D
-transfers in x, y, z, and the assembly of the chunk's integrals -- counted
-once per bra-function chunk, since every chunk kernel repeats the loop.
-Operations the compiler may remove (terms a chunk never uses) are counted only
-where the source skips them (zero solid-harmonic coefficients).
-f_contr: ket solid-harmonic transform and J/K digestion per shell quartet.
-The ket-primitive work of the outer loop is not counted (it is amortised over
-the bra primitives), so the figure is a lower bound of what executes.
+The counts follow csrc/eri_core.hpp (contract_chunk, rys_roots, rys_1d_x,
+ket_to_sph) and csrc/eri_class.hpp (k_eri_jk_seg's digestion) operation by
+operation: FMA = 2, add / mul / hardware rcp / rsq = 1.  The chunk geometry
+-- the bra functions [Q0, Q1) and ket cartesians [K0, K1) that each kernel
+of a class evaluates -- is read from the compiled library (qfe_class_chunks,
+i.e. JKChunks of eri_class.hpp itself, including its LDS-row rule), so the
+model follows the kernels that run.  Every chunk kernel repeats the primitive
+loop, so f_prim sums over the chunks.
+
+f_prim, per primitive quartet of one chunk:
+  setup     PA, PQ (6), p + q (1), rsqrt with two Newton steps (10), rpq (1),
+            rho (2), T (6), prefactor (3), q/(p+q) and p/(p+q) (2) = 31.
+            1/(2p) and 1/(2q) are table loads (no flops).
+  roots     the cheapest branch of rys_roots (large-T Boys moments for one or
+            two roots, the large-T Hermite limit for three or more), so the
+            count is a lower bound of what executes.
+  per root  B00, B10, B01, the two root scalings (10), C00 and C00' in x, y,
+            z (12), the weight on Iz (1) = 23; the 2-D recursion and the two
+            1-D transfers in each direction (rys_1d_x); the chunk's assembly,
+            per (bra function, ket cartesian): 2 mul per product of the three
+            1-D integrals, 1 add (unit coefficient) or 1 FMA per product, and
+            1 add into the accumulator.
+f_contr, per shell quartet and chunk: the ket solid-harmonic transform
+(1 FMA per nonzero coefficient and ket function) and the digestion (the deg
+scaling and 6 FMAs per bra function x ket function), plus 20 for the task
+set-up.  The ket-primitive work of the outer loop (amortised over the bra
+primitives) is not counted.
 """
 from __future__ import annotations
 
@@ -23,9 +36,7 @@ from functools import lru_cache
 
 import numpy as np
 
-from .basis import cart_list, solid_harmonic_matrix
-
-ACC_MAX = 48  # must match kAccMax in csrc/eri_class.hpp
+from .basis import solid_harmonic_matrix
 
 
 def ncart(l):
@@ -42,20 +53,27 @@ def _sph(l, pure=True):
     return np.eye(ncart(l))
 
 
-def chunks(la, lb, lc, ld, pure=True):
-    nbra = nsph(la, pure) * nsph(lb, pure)
-    nketc = ncart(lc) * ncart(ld)
-    ch = max(1, min(ACC_MAX // nketc, nbra))
-    return [(q0, min(q0 + ch, nbra)) for q0 in range(0, nbra, ch)]
+def chunks(la, lb, lc, ld):
+    """[(Q0, Q1, K0, K1)] of the compiled J/K kernels of the class."""
+    import ctypes
+    from . import _lib
+    cap = 4096
+    out = np.zeros(4 * cap, dtype=np.int32)
+    n = _lib.lib().qfe_class_chunks(la, lb, lc, ld, cap, _lib.ptr(out, ctypes.c_int))
+    if n < 0:
+        raise ValueError(f"class ({la}{lb}|{lc}{ld}) is not compiled into the library")
+    if n > cap:
+        raise ValueError("more chunks than the query holds")
+    return [tuple(int(x) for x in out[4 * i:4 * i + 4]) for i in range(n)]
 
 
 def _onedim(la, lb, lc, ld):
-    """flops of rys_1d for one direction."""
+    """flops of rys_1d_x for one direction (EA = la + lb, FC = lc + ld)."""
     EA, FC = la + lb, lc + ld
     f = 0
     if EA >= 1:
         f += 1                      # g[1][0] = C00 * s
-    f += max(EA - 1, 0) * 4         # g[i+1][0] = C00 g + i*B10*g  (mul, mul, fma)
+    f += max(EA - 1, 0) * 4         # g[i+1][0] = C00 g + i B10 g   (mul, mul, fma)
     if FC >= 1:
         f += 1                      # g[0][1]
         f += EA * 4                 # g[i][1]
@@ -69,40 +87,40 @@ def _onedim(la, lb, lc, ld):
 
 def _roots(nr):
     if nr == 1:
-        return 10 + 2 + 10          # large-T Boys F0,F1 (frcp, frsq, mul) + u = F1*rcp(F0)
+        return 10 + 2 + 10          # large-T F0, F1 (rcp + rsqrt with Newton steps, mul) + u = F1 / F0
     if nr == 2:
-        return 10 + 10 + 6 + 30 + 10 + 14  # Boys F0..F3 (large T) + 2x2 moment solve
-    return 20 + 2 * nr               # large-T Hermite limit: frcp, frsq, 2n mul
+        return 10 + 10 + 6 + 30 + 10 + 14  # F0..F3 (large T) + the 2 x 2 moment solve
+    return 20 + 2 * nr               # large-T Hermite limit: rcp, rsqrt, 2n mul
+
+
+SETUP = 3 + 3 + 1 + 10 + 1 + 2 + 6 + 3 + 2   # 31
+PER_ROOT = 10 + 12 + 1                      # 23
 
 
 @lru_cache(maxsize=None)
 def class_flops(la, lb, lc, ld, pure=True):
-    """(f_prim, f_contr) for class (la lb | lc ld) as evaluated."""
+    """(f_prim, f_contr) for class (la lb | lc ld) as compiled."""
     L = la + lb + lc + ld
     nr = L // 2 + 1
     Sa, Sb, Sc, Sd = _sph(la, pure), _sph(lb, pure), _sph(lc, pure), _sph(ld, pure)
-    nketc = ncart(lc) * ncart(ld)
-    nkets = nsph(lc, pure) * nsph(ld, pure)
-    nsb = nsph(lb, pure)
-    setup = 3 + 3 + 1 + 10 + 1 + 2 + 5 + 1 + 3 + 2 + 20   # PA, PQ, p+q, rsq, rpq, rho, T, pref, qr/pr, 1/2p, 1/2q
-    per_root_common = 2 + 3 + 3 + 2 + 12          # B00, B10, B01, xa/xc, C00/C00' (x, y, z)
+    nsb, nsc, nsd = nsph(lb, pure), nsph(lc, pure), nsph(ld, pure)
+    ncd = ncart(ld)
     onedim = 3 * _onedim(la, lb, lc, ld)
-    f_prim = 0
-    for q0, q1 in chunks(la, lb, lc, ld, pure):
+    f_prim = f_contr = 0
+    for q0, q1, k0, k1 in chunks(la, lb, lc, ld):
+        nk = k1 - k0
         asm = 0
         for q in range(q0, q1):
             ma, mb = divmod(q, nsb)
             nnz = int(np.count_nonzero(Sa[ma])) * int(np.count_nonzero(Sb[mb]))
             ones = sum(1 for a in np.nonzero(Sa[ma])[0] for b in np.nonzero(Sb[mb])[0]
                        if abs(Sa[ma, a] * Sb[mb, b] - 1.0) < 1e-15)
-            # per ket cartesian: v = Ix*Iy*Iz (2 mul), s += v (1) or fma (2), acc += s (1)
-            asm += nketc * (nnz * 2 + ones * 1 + (nnz - ones) * 2 + 1)
-        f_prim += setup + _roots(nr) + nr * (per_root_common + onedim + asm)
-    # contracted: ket transform + digestion (6 fma per element) + deg scaling
-    nnz_ket = sum(int(np.count_nonzero(Sc[mc])) * int(np.count_nonzero(Sd[md]))
-                  for mc in range(Sc.shape[0]) for md in range(Sd.shape[0]))
-    nbra = nsph(la, pure) * nsph(lb, pure)
-    f_contr = nbra * (2 * nnz_ket + nkets * (1 + 12)) + 20
+            asm += nk * (nnz * 2 + ones * 1 + (nnz - ones) * 2 + 1)
+        f_prim += SETUP + _roots(nr) + nr * (PER_ROOT + onedim + asm)
+        # ket transform of this chunk's cartesians into every ket function
+        nnz_ket = sum(1 for mc in range(nsc) for md in range(nsd) for k in range(k0, k1)
+                      if Sc[mc, k // ncd] * Sd[md, k % ncd] != 0.0)
+        f_contr += (q1 - q0) * (2 * nnz_ket + nsc * nsd * (1 + 12)) + 20
     return f_prim, f_contr
 
 
@@ -123,3 +141,13 @@ def table(maxl=2, pure=True):
             fp, fc = class_flops(la, lb, lc, ld, pure)
             rows.append(((la, lb, lc, ld), fp, fc))
     return rows
+
+
+def write_table(path, maxl=3):
+    """The committed per-class table (markdown) that the bench and the roofline
+    use: class, chunks, f_prim, f_contr."""
+    lines = ["| class | chunks | f_prim | f_contr |", "|---|---|---|---|"]
+    for (la, lb, lc, ld), fp, fc in table(maxl):
+        lines.append(f"| ({la}{lb}\\|{lc}{ld}) | {len(chunks(la, lb, lc, ld))} | {fp} | {fc} |")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

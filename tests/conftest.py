@@ -26,3 +26,20 @@ def random_density(nbf, seed, scale=1.0):
     rng = np.random.Generator(np.random.PCG64(seed))
     A = rng.uniform(-1.0, 1.0, (nbf, nbf)) * scale
     return 0.5 * (A + A.T)
+
+
+def decaying_density(basis, seed, scale=1.0, width=0.5):
+    """A symmetric density-like matrix whose elements fall off with the
+    distance r of their two AO centres, P_ij = scale u_ij exp(-width r^2),
+    u uniform in (-1, 1) from PCG64(seed): blocks of distant shells are small,
+    as in a molecular density, so the density test screens realistically.
+    Reproducible from numpy alone."""
+    import numpy as np
+    off = basis.shell_offsets
+    cen = np.zeros((basis.nbf, 3))
+    for s, sh in enumerate(basis.shells):
+        cen[off[s]:off[s + 1]] = sh.center
+    r2 = np.sum((cen[:, None, :] - cen[None, :, :]) ** 2, axis=2)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A = rng.uniform(-1.0, 1.0, (basis.nbf, basis.nbf))
+    return scale * 0.5 * (A + A.T) * np.exp(-width * r2)

@@ -2,7 +2,8 @@
 electron counts) of the basis-set-exchange JSON files (orbital bases, and
 the def2-universal-jfit auxiliary basis of density-fitted J) that
 qdk-chemistry ships (cpp/src/qdk/chemistry/algorithms/microsoft/scf/resources/
-compressed/<name>.tar.gz, member basis/<name>.json) into data/basis/.
+compressed/<name>.tar.gz, member basis/<name>.json) into the package's
+qdk_chemistry_amd/data/basis/.
 
 Only `electron_shells` (and `ecp_electrons`) of the kept elements are copied;
 the values are data (exponents and contraction coefficients as strings,
@@ -40,4 +41,6 @@ def main(out_dir):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(os.path.dirname(__file__), "basis"))
+    here = os.path.dirname(os.path.abspath(__file__))
+    default = os.path.join(here, "..", "..", "qdk-chemistry_amd", "qdk_chemistry_amd", "data", "basis")
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.normpath(default))

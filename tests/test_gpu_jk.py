@@ -150,18 +150,94 @@ def test_jcd_slots_zeroed_between_builds(monkeypatch):
     assert np.max(np.abs(K1 - Kr)) < 1e-10
 
 
-def test_jk_repeatable():
-    """Two builds of the same density agree to rounding (atomic additions
-    make the order of the last bits vary)."""
+@pytest.mark.parametrize("mix", [False, True])
+def test_default_build_is_bitwise_repeatable(mix):
+    """use_atomics = False (the reference's default, core/scf.h:81-82):
+    repeated builds of one density agree bit for bit, as the reference's
+    ordered reduction of thread-local buffers does (libint2_direct.cpp:
+    576-592); the fixed-point sums stay within 1e-10 of the oracle."""
+    from oracle.oracle import OracleERI
     from qdk_chemistry_amd.basis import load_basis
-    from qdk_chemistry_amd.eri import create
+    from qdk_chemistry_amd.eri import ERIConfig, create
     from qdk_chemistry_amd.molecule import water_cluster
     mol = water_cluster(4, 13)
     b = load_basis("cc-pvdz", mol.numbers, mol.coords)
     P = random_density(b.nbf, 9, 0.3)
-    eng = create(b, mol)
+    eng = create(b, mol, ERIConfig(is_mix_precision=mix))
     J1, K1 = eng.build_JK(P)
-    J2, K2 = eng.build_JK(P)
-    print("repeat: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J2)), np.max(np.abs(K1 - K2))), flush=True)
-    assert np.max(np.abs(J1 - J2)) < 1e-13
-    assert np.max(np.abs(K1 - K2)) < 1e-13
+    for _ in range(3):
+        J2, K2 = eng.build_JK(P)
+        assert np.array_equal(J1, J2) and np.array_equal(K1, K2)
+    J3, K3 = create(b, mol, ERIConfig(is_mix_precision=mix)).build_JK(P)
+    assert np.array_equal(J1, J3) and np.array_equal(K1, K3)
+    Jr, Kr = OracleERI(b, mol).build_JK(P)
+    tol = 1e-8 if mix else 1e-10
+    assert np.max(np.abs(J1 - Jr)) < tol and np.max(np.abs(K1 - Kr)) < tol
+
+
+def test_atomic_build_matches_oracle_and_deterministic_build():
+    """use_atomics = True: FP64 atomics, same values to rounding."""
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 13)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = random_density(b.nbf, 10, 0.3)
+    Ja, Ka = create(b, mol, ERIConfig(use_atomics=True)).build_JK(P)
+    Jd, Kd = create(b, mol, ERIConfig(use_atomics=False)).build_JK(P)
+    Jr, Kr = OracleERI(b, mol).build_JK(P)
+    print("atomics vs fixed point: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(Ja - Jd)), np.max(np.abs(Ka - Kd))),
+          flush=True)
+    assert np.max(np.abs(Ja - Jd)) < 1e-12 and np.max(np.abs(Ka - Kd)) < 1e-12
+    assert np.max(np.abs(Ja - Jr)) < 1e-10 and np.max(np.abs(Ka - Kr)) < 1e-10
+
+
+@pytest.mark.parametrize("atomics,mix", [(False, False), (True, False), (False, True)])
+def test_column_tiled_exchange_rows(monkeypatch, atomics, mix):
+    """Exchange rows split into shell-aligned column tiles (the path that
+    lifts the LDS limit on nbf), forced at small nbf with QFE_KCOL_TILE: the
+    same J and K as whole rows, and the oracle's.  Under mixed precision the
+    tiled kernel evaluates the FP32 tasks in FP64, so it is held to the
+    mixed tolerance against the oracle only."""
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 11)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = random_density(b.nbf, 14, 0.3)
+    cfg = ERIConfig(use_atomics=atomics, is_mix_precision=mix)
+    J1, K1 = create(b, mol, cfg).build_JK(P)
+    monkeypatch.setenv("QFE_KCOL_TILE", "20")  # nbf 96: five or more tiles
+    J0, K0 = create(b, mol, cfg).build_JK(P)
+    Jr, Kr = OracleERI(b, mol).build_JK(P)
+    print("tiled vs whole rows: max|dJ| %.2e max|dK| %.2e" % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))),
+          flush=True)
+    tol = 1e-8 if mix else 1e-10
+    if not mix:
+        assert np.max(np.abs(J1 - J0)) < 1e-12 and np.max(np.abs(K1 - K0)) < 1e-12
+    assert np.max(np.abs(J0 - Jr)) < tol and np.max(np.abs(K0 - Kr)) < tol
+
+
+def test_primitive_screening_changes_nothing_visible(monkeypatch):
+    """Primitive quartets below the engine precision (eps / max|P|, the
+    reference's Libint2 setting, libint2_direct.cpp:279-287) are skipped by
+    default; QFE_PRIM_SCREEN=0 keeps them all.  Both agree with each other
+    and with the oracle (which keeps every primitive quartet)."""
+    from oracle.oracle import OracleERI
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import create
+    from qdk_chemistry_amd.molecule import water_cluster
+    from conftest import decaying_density
+    mol = water_cluster(8, 29)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = decaying_density(b, 15, 1.0)
+    J1, K1 = create(b, mol).build_JK(P)
+    monkeypatch.setenv("QFE_PRIM_SCREEN", "0")
+    J0, K0 = create(b, mol).build_JK(P)
+    Jr, Kr = OracleERI(b, mol).build_JK(P)
+    print("screened vs all primitive quartets: max|dJ| %.2e max|dK| %.2e"
+          % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))), flush=True)
+    assert np.max(np.abs(J1 - J0)) < 1e-12 and np.max(np.abs(K1 - K0)) < 1e-12
+    assert np.max(np.abs(J1 - Jr)) < 1e-10 and np.max(np.abs(K1 - Kr)) < 1e-10

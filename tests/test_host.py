@@ -147,8 +147,10 @@ def test_c_abi_rejects_invalid_basis():
 
 
 def test_eri_multiplexer_routes_like_the_reference():
-    """One engine: called once for J and K; two engines: J from the first with
-    omega 0, K from the second (eri_multiplexer.cpp:60-71)."""
+    """One engine: called once for J and K; two engines: J from the first
+    with omega 0 and no K, K from the second with no J (eri_multiplexer.cpp:
+    60-71); gradients split the same way unless a gradient engine is given
+    (:79-94)."""
     from qdk_chemistry_amd.eri import ERIMultiplexer
     calls = []
 
@@ -156,16 +158,27 @@ def test_eri_multiplexer_routes_like_the_reference():
         def __init__(self, tag):
             self.tag = tag
 
-        def build_JK(self, P, alpha, beta, omega):
-            calls.append((self.tag, omega))
-            return np.full_like(P, 1.0 if self.tag == "j" else 2.0), np.full_like(P, 3.0 if self.tag == "j" else 4.0)
+        def build_JK(self, P, alpha, beta, omega, J=True, K=True):
+            calls.append((self.tag, omega, J, K))
+            return (np.full_like(P, 1.0 if self.tag == "j" else 2.0) if J else None,
+                    np.full_like(P, 3.0 if self.tag == "j" else 4.0) if K else None)
+
+        def get_gradients(self, P, alpha, beta, omega, dJ=True, dK=True):
+            calls.append(("grad-" + self.tag, omega, dJ, dK))
+            return (np.full(3, 5.0) if dJ else None, np.full(3, 6.0) if dK else None)
 
     P = np.zeros((2, 2))
     J, K = ERIMultiplexer(Fake("j")).build_JK(P, 1.0, 0.0, 0.3)
-    assert calls == [("j", 0.3)] and J[0, 0] == 1.0 and K[0, 0] == 3.0
+    assert calls == [("j", 0.3, True, True)] and J[0, 0] == 1.0 and K[0, 0] == 3.0
     calls.clear()
     J, K = ERIMultiplexer(Fake("j"), Fake("k")).build_JK(P, 1.0, 0.0, 0.3)
-    assert calls == [("j", 0.0), ("k", 0.3)] and J[0, 0] == 1.0 and K[0, 0] == 4.0
+    assert calls == [("j", 0.0, True, False), ("k", 0.3, False, True)] and J[0, 0] == 1.0 and K[0, 0] == 4.0
+    calls.clear()
+    ERIMultiplexer(Fake("j"), Fake("k")).get_gradients(P, 1.0, 0.0, 0.3)
+    assert calls == [("grad-j", 0.0, True, False), ("grad-k", 0.3, False, True)]
+    calls.clear()
+    ERIMultiplexer(Fake("j"), Fake("k"), Fake("g")).get_gradients(P, 1.0, 0.0, 0.3)
+    assert calls == [("grad-g", 0.3, True, True)]
 
 
 _INT1E = [(0, "overlap", 1e-9), (1, "kinetic", 1e-9), (2, "nuclear", 1e-8)]
