@@ -1,7 +1,8 @@
 """Fock-build benchmark: (H2O)64 / cc-pVDZ (1536 basis functions) on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--system h2o64|h2o16|c20h42]
-                    [--no-scf] [--no-cpu] [--no-mixed]
+                    [--no-scf] [--no-cpu] [--no-mixed] [--no-grad] [--no-atomics]
+                    [--density P.npy] [--save-density P.npy]
 
 One step is one full J/K build (screened task list + ERI class kernels +
 digestion + symmetrisation) of the seeded (H2O)64 cluster at the converged
@@ -95,9 +96,12 @@ def summarize(run, steps):
     f64 = f32 = 0.0
     dom = None
     dom_total = -1.0
+    classes = {}
     for (prec, cls), (nq, npq, ms, n) in run["per_class"].items():
         fp, fc = class_flops(*cls)
         f = npq * fp + nq * fc
+        classes["(%d%d|%d%d) %s" % (*cls, "fp32" if prec else "fp64")] = dict(
+            quartets=nq / steps, prim_quartets=npq / steps, ms=ms / steps, flops=f / steps)
         if prec == 0:
             f64 += f
             if ms > dom_total:  # the FP64 class kernel group with the most time
@@ -113,7 +117,7 @@ def summarize(run, steps):
                 eri_kernels_ms=eri_ms, screen_ms=run["stats"]["ms_screen"],
                 flops_fp64_per_build=f64 / steps, flops_fp32_per_build=f32 / steps,
                 peak_fraction_eri=t_peak / (eri_ms * 1e-3) if eri_ms > 0 else 0.0,
-                dominant=dom)
+                classes=classes, dominant=dom)
 
 
 def calibration_copy(eng, nbf):
@@ -262,6 +266,31 @@ def dfj_timings(nwat):
     return out
 
 
+def gradient_timings(systems=("h2o16", "c20h42")):
+    """The ERI gradient (adjoint kernels) of the gradient configurations,
+    (H2O)16/cc-pVDZ and C20H42/cc-pVTZ (f shells: the generic kernel), at
+    the atomic-guess density: one warm-up, then one timed call each."""
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.scf import atom_guess
+    out = {}
+    for name in systems:
+        mol, bn, desc = system(name)
+        b = load_basis(bn, mol.numbers, mol.coords)
+        eng = create(b, mol, ERIConfig(eri_threshold=1e-12, shell_pair_threshold=1e-12))
+        P = atom_guess(mol, bn)
+        eng.get_gradients(P)
+        t = time.perf_counter()
+        eng.get_gradients(P)
+        dt = time.perf_counter() - t
+        st = eng.stats()
+        eng.close()
+        out[name] = dict(workload=desc, nbf=b.nbf, density="atomic guess", seconds=dt, quartets=st["quartets"],
+                         quartets_per_s=st["quartets"] / dt, ms_kernels=st["ms_eri"])
+        log(f"ERI gradient {desc}: {dt * 1e3:.1f} ms, {st['quartets']:.4g} quartets, kernels {st['ms_eri']:.1f} ms")
+    return out
+
+
 def cpu_baseline(system_name, densities, stride, repeats=5):
     """The oracle's build_JK on a bounded sample of the same build, run in a
     child process (oracle/cpu_baseline.py) so that OMP_NUM_THREADS and
@@ -304,6 +333,8 @@ def main():
     ap.add_argument("--no-scf", action="store_true", help="time at the atomic-guess density")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="skip the ERI gradient timing")
+    ap.add_argument("--no-atomics", action="store_true", help="skip the use_atomics = True builds")
     ap.add_argument("--cpu-stride", type=int, default=96)
     ap.add_argument("--mo-waters", type=int, default=4, help="water cluster of the MO-transform timing (0: skip)")
     ap.add_argument("--mo-nt", type=int, default=48)
@@ -427,22 +458,28 @@ def main():
             log(f"atomic-guess density, mixed: {sgm['ms_per_build']:.2f} ms/build "
                 f"({guess['mixed']['speedup_vs_fp64']:.2f}x)")
     # ERI nuclear gradient (the adjoint kernel) at the same density
-    eng.get_gradients(P)
-    t = time.perf_counter()
-    eng.get_gradients(P)
-    gst = eng.stats()
-    grad = dict(seconds=time.perf_counter() - t, quartets=gst["quartets"], ms_screen=gst["ms_screen"],
-                ms_kernels=gst["ms_eri"])
-    log(f"ERI gradient: {grad['seconds'] * 1e3:.1f} ms ({gst['quartets']:.4g} quartets, kernels {gst['ms_eri']:.1f} ms)")
+    grad = None
+    if not args.no_grad:
+        eng.get_gradients(P)
+        t = time.perf_counter()
+        eng.get_gradients(P)
+        gst = eng.stats()
+        grad = dict(seconds=time.perf_counter() - t, quartets=gst["quartets"], ms_screen=gst["ms_screen"],
+                    ms_kernels=gst["ms_eri"])
+        log(f"ERI gradient: {grad['seconds'] * 1e3:.1f} ms ({gst['quartets']:.4g} quartets, "
+            f"kernels {gst['ms_eri']:.1f} ms)")
     # the same FP64 builds with use_atomics = True (double-precision atomics:
     # no fixed-point conversion, not bitwise repeatable), on a fresh engine
     eng.close()
-    eng_a = create(basis, mol, ERIConfig(eri_threshold=1e-12, shell_pair_threshold=1e-12, profile=True,
-                                         use_atomics=True))
-    sa = summarize(timed_builds(eng_a, P, args.steps, args.warmup), args.steps)
-    eng_a.close()
-    atomics = {k: v for k, v in sa.items() if k != "dominant"}
-    log(f"use_atomics=True, FP64: {sa['ms_per_build']:.2f} ms/build, {sa['quartets_per_s']:.4g} quartets/s")
+    atomics = None
+    if not args.no_atomics:
+        eng_a = create(basis, mol, ERIConfig(eri_threshold=1e-12, shell_pair_threshold=1e-12, profile=True,
+                                             use_atomics=True))
+        sa = summarize(timed_builds(eng_a, P, args.steps, args.warmup), args.steps)
+        eng_a.close()
+        atomics = {k: v for k, v in sa.items() if k != "dominant"}
+        log(f"use_atomics=True, FP64: {sa['ms_per_build']:.2f} ms/build, {sa['quartets_per_s']:.4g} quartets/s, "
+            f"ERI kernels {sa['eri_kernels_ms']:.2f} ms")
     out = {
         "metric": METRIC,
         "value": headline["quartets_per_s"],
@@ -470,6 +507,8 @@ def main():
     out["config"].update(result)
     if "scf_fp64" in result:
         out["config"]["harness"] = harness_timings(basis, mol, P, not args.no_cpu)
+    if not args.no_grad:
+        out["config"]["eri_gradient_configs"] = gradient_timings()
     if args.dfj_waters > 0:
         out["config"]["dfj"] = dfj_timings(args.dfj_waters)
     if args.mo_waters > 0:

@@ -288,6 +288,8 @@ struct Engine {
   double* jbuf_p = nullptr;
   unsigned long long* jmask_p = nullptr;
   DevBuf<int> sh_bf0_d, sh_nf_d;
+  DevBuf<int4> kets_d;        // per pair {prim0, nprim, sa, sb} (class kernels' ket side)
+  DevBuf<double4> shells_d;   // per shell {x, y, z, first AO}
   DevBuf<double> Pn_d, pmax_d;  // shell-block density norms (nsh x nsh) and their maximum
   DevBuf<int> bad_d;
   DevBuf<double> P_d, J_d, K_d;      // device buffers handed out by qfe_device_buffers (SCF context too)
@@ -588,6 +590,17 @@ struct Engine {
     std::vector<int2> psh(npair);
     for (int i = 0; i < npair; ++i) psh[i] = make_int2(pairs_h[i].sa, pairs_h[i].sb);
     psh_d.upload(psh, stream);
+    {
+      std::vector<int4> kr(npair);
+      for (int i = 0; i < npair; ++i)
+        kr[i] = make_int4(pairs_h[i].prim0, pairs_h[i].nprim, pairs_h[i].sa, pairs_h[i].sb);
+      kets_d.upload(kr, stream);
+      std::vector<double4> sr(nsh);
+      for (int s2 = 0; s2 < nsh; ++s2)
+        sr[s2] = make_double4(basis.shells[s2].O[0], basis.shells[s2].O[1], basis.shells[s2].O[2],
+                              (double)basis.bf0[s2]);
+      shells_d.upload(sr, stream);
+    }
     Pcd_d.alloc(npair);
     ppos_d.upload(ppos2, stream);
     pk_d.upload(pk2, stream);
@@ -981,7 +994,7 @@ struct Engine {
                    offsets_d.p + (prec ? (long long)items_h.size() : 0), tasks_p, lds_max,
                    jslots ? jbuf_p : nullptr, jslot_d.p, qpos_d.p, seg_min_waves,
                    pmax_d.p, prim_screen ? 1 : 0, det ? 1 : 0, Jx, Kx, fxo_d.p, sh_bf0_h.data(), nsh,
-                   kcol_tile};
+                   kcol_tile, kets_d.p, shells_d.p};
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
         if (e) HIP_CHECK(hipEventRecord(e[0], s));
         launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);

@@ -128,6 +128,30 @@ struct SegRows {
   static constexpr int NR = NMA + NMB;  // LDS rows: a functions, then b functions
 };
 
+// The ket pair of a task from its 16-byte record {prim0, nprim, sa, sb} and
+// the shell table {x, y, z, first AO} (nsh entries, small enough to stay in
+// L2): 16 bytes per task from the pair table instead of the 80-byte header
+QFE_DI PairHdr ket_record(const int4* __restrict__ kets, const double4* __restrict__ shells, int p) {
+  const int4 r = kets[p];
+  const double4 c = shells[r.z], d = shells[r.w];
+  PairHdr h;
+  h.A[0] = c.x;
+  h.A[1] = c.y;
+  h.A[2] = c.z;
+  h.B[0] = d.x;
+  h.B[1] = d.y;
+  h.B[2] = d.z;
+  h.prim0 = r.x;
+  h.nprim = r.y;
+  h.sa = r.z;
+  h.sb = r.w;
+  h.bfa = (int)c.w;
+  h.bfb = (int)d.w;
+  h.cls = 0;
+  h.pad = 0;
+  return h;
+}
+
 // P[e] for an element index e < 2^29 (nbf < 23170, checked on the host): a
 // 32-bit byte offset from the uniform base, which the load can take in its
 // scalar-base form (no 64-bit address per lane)
@@ -208,7 +232,10 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
   const double dbra = bra.sa == bra.sb ? 1.0 : 2.0;
   for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
     const int2 tk = tasks[t];
-    const PairHdr ket = pairs[tk.y];
+    // classes with an f shell are bound by their integrals, not by this
+    // load: they read the full header (the compact record costs them registers)
+    constexpr bool FCLS = C::LA > 2 || C::LB > 2 || C::LC > 2 || C::LD > 2;
+    const PairHdr ket = FCLS ? pairs[tk.y] : ket_record(ex.kets, ex.shells, tk.y);
     const int k0 = ket.bfa, l0 = ket.bfb;
     // with column tiles a task belongs to the tiles of its ket shells: its
     // c-column updates and its Coulomb terms go with the tile of c, its
@@ -218,9 +245,6 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
     if (!inC && !inD) continue;
     R acc[NQ][K1 - K0];
     contract_chunk<C, Q0, Q1, R, K0, K1>(bra, ket, ppos, pk, ra, acc, lnp);
-    // keep the digestion's density loads after the contraction (hoisted
-    // above it they hold registers through the whole primitive loop)
-    asm volatile("" ::: "memory");
     const double deg = dbra * (ket.sa == ket.sb ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
     const double degj = inC ? deg : 0.0;
     double Pkl[C::NKETS], jkl[C::NKETS];
@@ -497,7 +521,7 @@ struct ClassOps {
     if (a.nseg <= 0 || a.segs == nullptr) throw std::runtime_error("segment list missing");
     const size_t lim = (size_t)(a.lds_max < 0 ? -a.lds_max : a.lds_max);
     const bool det = a.det != 0;
-    SegExtra ex{a.pmax, a.prim_screen, a.det, a.Jx, a.Kx, a.fx_over, 1, {0}, {a.nbf}};
+    SegExtra ex{a.pmax, a.prim_screen, a.det, a.Jx, a.Kx, a.fx_over, a.kets, a.shells, 1, {0}, {a.nbf}};
     const int cap = a.kcol_tile > 0 ? a.kcol_tile : a.nbf;
     if (cap >= a.nbf) {
       const size_t tot_k = seg_total<Q0, Q1>(a.nbf, false, det);

@@ -334,14 +334,6 @@ QFE_DI void rys_1d(R s, R C00, R C00p, R B00, R B10, R B01, R AB, R CD, OneD<C, 
   rys_1d_x<C::EA, C::FC, C::LA, C::LB, C::LC, C::LD, R>(s, C00, C00p, B00, B10, B01, AB, CD, out);
 }
 
-// Contract all primitive quartets of (bra|ket) into acc[q][k] for the bra
-// function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and the ket
-// cartesian pairs K0 + k, k < K1 - K0 (row-major over (c, d)), in precision R.
-// Primitive quartets whose prefactor bound exp(sig_ab + sig_cd) is below
-// exp(lnp) are skipped, as the reference's Libint2 engine skips those below
-// its precision (ScreeningMethod::Original at eps / max|P|,
-// libint2_direct.cpp:279-292); the primitives of a pair are sorted by sig,
-// so the kept ones form a prefix of both loops.  lnp = -inf keeps all.
 // ln of the primitive-quartet cut.  FP64: the reference's engine precision
 // max(eps / max|P|, eps / 1e10) (libint2_direct.cpp:279-287).  FP32 tasks
 // (Q_ab Q_cd below the mixed threshold, 1e-6) carry ~1e-13 absolute rounding
@@ -356,6 +348,14 @@ QFE_DI R prim_ln_cut(const double* pmax, int prim_screen) {
   return (R)log(prec);
 }
 
+// Contract all primitive quartets of (bra|ket) into acc[q][k] for the bra
+// function pairs q = Q0 .. Q1-1 (row-major over (ma, mb)) and the ket
+// cartesian pairs K0 + k, k < K1 - K0 (row-major over (c, d)), in precision R.
+// Primitive quartets whose prefactor bound exp(sig_ab + sig_cd) is below
+// exp(lnp) are skipped, as the reference's Libint2 engine skips those below
+// its precision (ScreeningMethod::Original at eps / max|P|,
+// libint2_direct.cpp:279-292); the primitives of a pair are sorted by sig,
+// so the kept ones form a prefix of both loops.  lnp = -inf keeps all.
 template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC>
 QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
                            const typename Vec4<R>::type* __restrict__ ppos, const R* __restrict__ pk,

@@ -25,6 +25,8 @@ struct SegExtra {
   unsigned long long* Jx;   // (hi, lo) pairs per J element, deterministic builds
   unsigned long long* Kx;   // (hi, lo) pairs per K element, deterministic builds
   int* fx_over;             // set to 1 if a value leaves the fixed-point range
+  const int4* kets;         // per pair {prim0, nprim, sa, sb}: the ket side of a task
+  const double4* shells;    // per shell {x, y, z, first AO}
   int ntile;                // column tiles of the exchange rows (1: whole rows)
   int tile_lo[kMaxColTiles], tile_hi[kMaxColTiles];  // [lo, hi) columns of each tile (shell-aligned)
 };
@@ -73,6 +75,8 @@ struct JKLaunch {
   const int* sh_bf0;        // host: first function of each shell, nsh + 1 entries (tile boundaries)
   int nsh;
   int kcol_tile;            // > 0: exchange-row tiles of at most this many columns (testing knob)
+  const int4* kets;
+  const double4* shells;
 };
 
 struct GradLaunch {

@@ -158,8 +158,10 @@ class RysERI:
         cap = 256
         cls = np.zeros(4 * cap, dtype=np.int32)
         q, pq, epq, ms = np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        # the executed count runs a counting kernel over the last task list:
+        # only when asked for
         n = _lib.lib().qfe_class_stats(self.h, precision, cap, _lib.ptr(cls, ctypes.c_int), _lib.ptr(q),
-                                       _lib.ptr(pq), _lib.ptr(epq), _lib.ptr(ms))
+                                       _lib.ptr(pq), _lib.ptr(epq) if executed else None, _lib.ptr(ms))
         p = epq if executed else pq
         return [(tuple(int(x) for x in cls[4 * i:4 * i + 4]), float(q[i]), float(p[i]), float(ms[i]))
                 for i in range(min(n, cap))]
