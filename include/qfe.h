@@ -97,6 +97,20 @@ int qfe_build_jk(qfe_handle* h, const double* P, double* J, double* K, double al
 int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK, double alpha,
                         double beta, double omega, void* stream);
 
+/* qfe_build_jk_device without any device allocation in the call.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-build task list, J_cd share slots and survivor masks
+ * live in a fixed device array of qfe_noalloc_scratch_bytes() (256 MiB) per
+ * GPU, held until the process ends; a build that needs more returns
+ * QFE_ERR_INVALID (nothing is computed).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race, and the array keeps the last
+ * call's contents (each build clears what it needs).  Arguments and results
+ * are those of qfe_build_jk_device. */
+int qfe_build_jk_device_noalloc(qfe_handle* h, const double* dP, double* dJ, double* dK, double alpha,
+                                double beta, double omega, void* stream);
+unsigned long long qfe_noalloc_scratch_bytes(void);
+
 /* Engine-owned HBM buffers of nbf x nbf doubles for callers that keep P, J
  * and K resident on the device (e.g. to time qfe_build_jk_device without
  * PCIe traffic), and a synchronous copy on the engine's stream

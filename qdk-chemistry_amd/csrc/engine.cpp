@@ -67,6 +67,10 @@ struct Unsupported : std::runtime_error {
       throw ::qfe::DeviceError(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x);  \
   } while (0)
 
+// Set while the no-alloc entry point runs on this thread: a DevBuf that would
+// have to grow then throws instead of calling hipMalloc.
+thread_local bool g_no_alloc = false;
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -83,6 +87,7 @@ struct DevBuf {
   }
   void alloc(size_t count) {
     if (count <= n && p) return;
+    if (g_no_alloc) throw std::logic_error("a device allocation inside the no-alloc entry point");
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
@@ -187,6 +192,30 @@ void rys_host(int n, double T, double* u, double* w) {
 }
 
 // ------------------------------------------------------------ the engine
+// ------------------------------------------------------------ no-alloc scratch
+// Per-build buffers of qfe_build_jk_device_noalloc (the task list, the J_cd
+// share slots and their survivor masks): a file-scope device array of fixed
+// size, so that the build itself allocates nothing.  Every GPU the library
+// runs on holds its own copy, at its own address, until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (the array's name is not a device address there).
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// the device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current)
+static unsigned char* noalloc_scratch(int device) {
+  static std::mutex mu;
+  static std::map<int, unsigned char*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void* p = nullptr;
+  HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw DeviceError("hipGetSymbolAddress returned no address for the no-alloc scratch");
+  return addr.emplace(device, static_cast<unsigned char*>(p)).first->second;
+}
+
 struct Engine {
   // input
   BasisHost basis;
@@ -796,6 +825,7 @@ struct Engine {
     Pn_d.alloc((size_t)nsh * nsh);
     pmax_d.alloc(1);
     bad_d.alloc(1);
+    scratch_ptr = noalloc_scratch(device);
     build_pairs();
     const int ncp = (int)cp_bra.size();
     cls_ms.assign(2 * (size_t)ncp, 0.0);
@@ -804,6 +834,8 @@ struct Engine {
       for (auto& e : cls_ev) HIP_CHECK(hipEventCreate(&e));
     }
   }
+
+  unsigned char* scratch_ptr = nullptr;  // this device's no-alloc scratch (device address)
 
   // Device memory a build may take its per-build buffers from instead of the
   // engine's own (the no-alloc entry point): base address and capacity.
@@ -1357,6 +1389,27 @@ int qfe_build_jk_device(qfe_handle* h, const double* dP, double* dJ, double* dK,
     h->e.build_device(dP, dJ, dK, alpha + beta, stream ? (hipStream_t)stream : h->e.stream);
   });
 }
+
+int qfe_build_jk_device_noalloc(qfe_handle* h, const double* dP, double* dJ, double* dK, double alpha, double beta,
+                                double omega, void* stream) {
+  return guarded([&] {
+    if (!h) throw std::runtime_error("ERI engine is not initialized");
+    if (!dP) throw std::invalid_argument("null density");
+    if (std::fabs(omega) > 1e-12)
+      throw qfe::Unsupported("range-separated exchange (omega != 0) is not supported by this engine");
+    auto& e = h->e;
+    HIP_CHECK(hipSetDevice(e.device));
+    if (!e.scratch_ptr) throw std::runtime_error("no-alloc scratch not set up on this device");
+    const qfe::Engine::Scratch ext{e.scratch_ptr, qfe::kNoAllocScratchBytes};
+    struct Guard {
+      Guard() { qfe::g_no_alloc = true; }
+      ~Guard() { qfe::g_no_alloc = false; }
+    } guard;
+    e.build_device(dP, dJ, dK, alpha + beta, stream ? (hipStream_t)stream : e.stream, &ext);
+  });
+}
+
+unsigned long long qfe_noalloc_scratch_bytes(void) { return (unsigned long long)qfe::kNoAllocScratchBytes; }
 
 int qfe_device_buffers(qfe_handle* h, double** dP, double** dJ, double** dK) {
   return guarded([&] {

@@ -43,7 +43,7 @@ EXPORTS = ["qfe_create", "qfe_build_jk", "qfe_build_jk_device", "qfe_gradients",
            "qfe_one_body_grad", "qfe_one_body_gpu", "qfe_one_body_grad_gpu", "qfe_eigh_gpu",
            "qfe_dfj_create", "qfe_dfj_build_j", "qfe_dfj_naux", "qfe_dfj_destroy",
            "qfe_quarter_trans", "qfe_mo_eri", "qfe_scf_create", "qfe_scf_set_density",
-           "qfe_scf_step", "qfe_scf_update", "qfe_scf_finish", "qfe_scf_get", "qfe_scf_destroy", "qfe_get_stats", "qfe_class_stats", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_class_chunks", "qfe_nbf", "qfe_max_l", "qfe_destroy",
+           "qfe_scf_step", "qfe_scf_update", "qfe_scf_finish", "qfe_scf_get", "qfe_scf_destroy", "qfe_get_stats", "qfe_class_stats", "qfe_build_jk_device_noalloc", "qfe_noalloc_scratch_bytes", "qfe_set_precision", "qfe_device_buffers", "qfe_copy", "qfe_class_chunks", "qfe_nbf", "qfe_max_l", "qfe_destroy",
            "qfe_last_error"]
 
 _lib = None
@@ -68,6 +68,9 @@ def lib():
                                    ctypes.c_double, ctypes.c_int]
         L.qfe_build_jk_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        L.qfe_build_jk_device_noalloc.argtypes = L.qfe_build_jk_device.argtypes
+        L.qfe_noalloc_scratch_bytes.argtypes = []
+        L.qfe_noalloc_scratch_bytes.restype = ctypes.c_ulonglong
         L.qfe_gradients.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int]
         L.qfe_one_body.argtypes = [ctypes.POINTER(QfeBasis), ctypes.c_int, _dp]
