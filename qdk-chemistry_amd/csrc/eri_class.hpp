@@ -165,13 +165,15 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
                                                     const int2* __restrict__ tasks,
                                                     const PairHdr* __restrict__ pairs,
                                                     const typename Vec4<R>::type* __restrict__ ppos,
-                                                    const R* __restrict__ pk, RysArgsT<R> ra,
+                                                    const R* __restrict__ pk, const R* __restrict__ sig,
+                                                    RysArgsT<R> ra,
                                                     const double* __restrict__ P, double* __restrict__ J,
                                                     double* __restrict__ K, int nbf, double kscale,
                                                     double* __restrict__ jbuf, const long long* __restrict__ jslot,
                                                     const int2* __restrict__ qpos, int jfirst, int nsplit,
                                                     SegExtra ex) {
   using S = SegRows<C, Q0, Q1>;
+  constexpr bool FCLS = C::LA > 2 || C::LB > 2 || C::LC > 2 || C::LD > 2;  // a class with an f shell
   extern __shared__ double lds[];
   // block = (segment, part, column tile): a long segment may be split over
   // nsplit blocks, each with its own LDS rows, when there are too few
@@ -234,7 +236,6 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
     const int2 tk = tasks[t];
     // classes with an f shell are bound by their integrals, not by this
     // load: they read the full header (the compact record costs them registers)
-    constexpr bool FCLS = C::LA > 2 || C::LB > 2 || C::LC > 2 || C::LD > 2;
     const PairHdr ket = FCLS ? pairs[tk.y] : ket_record(ex.kets, ex.shells, tk.y);
     const int k0 = ket.bfa, l0 = ket.bfb;
     // with column tiles a task belongs to the tiles of its ket shells: its
@@ -244,7 +245,10 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
     const bool inD = !TILED || (unsigned)(l0 - c0) < (unsigned)W;
     if (!inC && !inD) continue;
     R acc[NQ][K1 - K0];
-    contract_chunk<C, Q0, Q1, R, K0, K1>(bra, ket, ppos, pk, ra, acc, lnp);
+    // the wave-uniform bra loop, except in classes with an f shell (their
+    // long primitive bodies hide the per-lane break's wait, and the uniform
+    // loop costs them registers)
+    contract_chunk<C, Q0, Q1, R, K0, K1, !FCLS>(bra, ket, ppos, pk, ra, acc, lnp, sig);
     const double deg = dbra * (ket.sa == ket.sb ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
     const double degj = inC ? deg : 0.0;
     double Pkl[C::NKETS], jkl[C::NKETS];
@@ -494,10 +498,12 @@ struct ClassOps {
     const dim3 grid((unsigned)((long long)a.nseg * nsplit * ex.ntile));
     if constexpr (sizeof(R) == 4)
       hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos32, a.pk32,
-                         *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst, nsplit, ex);
+                         a.ra32->sig, *a.ra32, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst,
+                         nsplit, ex);
     else
       hipLaunchKernelGGL(kern, grid, dim3(SEG_BLK), bytes, s, a.segs, a.offs, a.tasks_all, a.pairs, a.ppos, a.pk,
-                         *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst, nsplit, ex);
+                         a.ra->sig, *a.ra, a.P, a.J, a.K, a.nbf, a.kscale, a.jbuf, a.jslot, a.qpos, jfirst,
+                         nsplit, ex);
     return true;
   }
 

@@ -356,10 +356,20 @@ QFE_DI R prim_ln_cut(const double* pmax, int prim_screen) {
 // its precision (ScreeningMethod::Original at eps / max|P|,
 // libint2_direct.cpp:279-292); the primitives of a pair are sorted by sig,
 // so the kept ones form a prefix of both loops.  lnp = -inf keeps all.
-template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC>
+// UNIFORM: the bra loop ends on a wave-wide vote instead of a per-lane break
+// (sig must then be a __restrict__ kernel parameter, passed as sigp).  The
+// bra primitive index is the same on every lane; with a per-lane break the
+// loop exit is divergent and the compiler fetches sig with a vector load
+// waited on in every bra iteration, and sig read through the argument struct
+// is not known to be unchanged by the kernel's own stores.  With the vote,
+// readfirstlane and the restrict pointer the bra data are scalar loads; a
+// lane past its cut runs on with a zero prefactor (in SIMT a masked lane
+// would cost the same).
+template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC, bool UNIFORM = false>
 QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
                            const typename Vec4<R>::type* __restrict__ ppos, const R* __restrict__ pk,
-                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][K1 - K0], R lnp = R(-3.0e38)) {
+                           const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][K1 - K0], R lnp = R(-3.0e38),
+                           const R* __restrict__ sigp = nullptr) {
   constexpr int NK = K1 - K0;
   const R TWO_PI_52 = R(34.98683665524972);  // 2 pi^(5/2)
   sfor<Q1 - Q0>([&](auto q) { sfor<NK>([&](auto k) { acc[q][k] = R(0); }); });
@@ -367,7 +377,7 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
   const R CDx = R(ket.A[0] - ket.B[0]), CDy = R(ket.A[1] - ket.B[1]), CDz = R(ket.A[2] - ket.B[2]);
   const R Ax = R(bra.A[0]), Ay = R(bra.A[1]), Az = R(bra.A[2]);
   const R Cx = R(ket.A[0]), Cy = R(ket.A[1]), Cz = R(ket.A[2]);
-  const R* __restrict__ sig = ra.sig;
+  const R* __restrict__ sig = UNIFORM ? sigp : ra.sig;
   const R sb0 = sig[bra.prim0];  // the bra's largest factor
   // ket primitives outer (per-lane data), bra primitives inner: the bra pair
   // is usually shared by the whole wave, so the inner loads are broadcasts
@@ -380,10 +390,18 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
     const R q = Q.x;
     const R QCx = Q.y - Cx, QCy = Q.z - Cy, QCz = Q.w - Cz;
     const R h2q = ra.h2[ik];
-    for (int ib = bra.prim0; ib < bra.prim0 + bra.nprim; ++ib) {
-      if (sig[ib] < blim) break;  // this and every later bra primitive
+    for (int ib0 = bra.prim0; ib0 < bra.prim0 + bra.nprim; ++ib0) {
+      int ib = ib0;
+      bool on = true;
+      if constexpr (UNIFORM) {
+        ib = __builtin_amdgcn_readfirstlane(ib0);
+        on = !(sig[ib] < blim);
+        if (__ballot(on) == 0ull) break;  // past the cut on every lane
+      } else {
+        if (sig[ib] < blim) break;  // this and every later bra primitive
+      }
       const auto P = ppos[ib];
-      const R Kb = pk[ib];
+      const R Kb = on ? pk[ib] : R(0);
       const R p = P.x;
       const R PAx = P.y - Ax, PAy = P.z - Ay, PAz = P.w - Az;
       const R PQx = P.y - Q.y, PQy = P.z - Q.z, PQz = P.w - Q.w;
