@@ -85,16 +85,34 @@ def _onedim(la, lb, lc, ld):
     return f
 
 
-def _roots(nr):
+def _roots(nr, L=1):
     if nr == 1:
+        if L == 0:
+            return 9                # F0 only (rsqrt with Newton steps, mul): no root is needed
         return 10 + 2 + 10          # large-T F0, F1 (rcp + rsqrt with Newton steps, mul) + u = F1 / F0
     if nr == 2:
         return 10 + 10 + 6 + 30 + 10 + 14  # F0..F3 (large T) + the 2 x 2 moment solve
     return 20 + 2 * nr               # large-T Hermite limit: rcp, rsqrt, 2n mul
 
 
-SETUP = 3 + 3 + 1 + 10 + 1 + 2 + 6 + 3 + 2   # 31
-PER_ROOT = 10 + 12 + 1                      # 23
+SETUP = 3 + 3 + 1 + 10 + 1 + 2 + 6 + 3 + 2   # 31 with every term used
+PER_ROOT = 10 + 12 + 1                      # 23 with every term used
+
+
+def _setup(EA, FC):
+    """The primitive set-up terms the class uses (the compiler drops the rest):
+    PA (3) with a bra shell above s, PQ (3), p + q (1), rsqrt with two Newton
+    steps (10), rpq (1), rho (2), T (6), prefactor (3), q/(p+q) and p/(p+q)
+    (1 each) where a root scaling uses them."""
+    return (3 if EA else 0) + 3 + 1 + 10 + 1 + 2 + 6 + 3 + (1 if EA else 0) + (1 if FC else 0)
+
+
+def _per_root(EA, FC):
+    """Per root: B00 = rpq u / 2 (2) for bra and ket above s; B10 (3) from EA = 2,
+    B01 (3) from FC = 2; the root scalings q/(p+q) u (1) and p/(p+q) u (1);
+    C00 (3 FMA) and C00' (3 FMA) where needed; the weight on Iz (1)."""
+    return ((2 if EA and FC else 0) + (3 if EA >= 2 else 0) + (3 if FC >= 2 else 0) +
+            (1 + 6 if EA else 0) + (1 + 6 if FC else 0) + 1)
 
 
 @lru_cache(maxsize=None)
@@ -116,7 +134,8 @@ def class_flops(la, lb, lc, ld, pure=True):
             ones = sum(1 for a in np.nonzero(Sa[ma])[0] for b in np.nonzero(Sb[mb])[0]
                        if abs(Sa[ma, a] * Sb[mb, b] - 1.0) < 1e-15)
             asm += nk * (nnz * 2 + ones * 1 + (nnz - ones) * 2 + 1)
-        f_prim += SETUP + _roots(nr) + nr * (PER_ROOT + onedim + asm)
+        EA, FC = la + lb, lc + ld
+        f_prim += _setup(EA, FC) + _roots(nr, L) + nr * (_per_root(EA, FC) + onedim + asm)
         # ket transform of this chunk's cartesians into every ket function
         nnz_ket = sum(1 for mc in range(nsc) for md in range(nsd) for k in range(k0, k1)
                       if Sc[mc, k // ncd] * Sd[md, k % ncd] != 0.0)
