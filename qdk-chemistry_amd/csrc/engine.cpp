@@ -334,7 +334,23 @@ struct Engine {
 
   qfe_stats stats{};
 
+  // extra streams for the class kernels of one build (default priority):
+  // kernels of different classes run concurrently and fill each other's
+  // tails; J and K take their contributions through atomics (integer ones in
+  // deterministic builds, so the result does not depend on the order) and
+  // every class writes its own J_cd share slots, so the classes are
+  // independent.  The chunks of one class stay in order on one stream.
+  // Profiled engines launch on one stream (clean per-class times).
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> side_ev;
+  hipEvent_t fork_ev = nullptr;
+
   ~Engine() {
+    for (auto& e : side_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (auto& st : side)
+      if (st) (void)hipStreamDestroy(st);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : cls_ev)
@@ -800,6 +816,16 @@ struct Engine {
     if (const char* e = std::getenv("QFE_KCOL_TILE")) kcol_tile = std::max(0, std::atoi(e));
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    // QFE_STREAMS=<n>: streams the class kernels of a build spread over (default 4)
+    {
+      int ns = 4;
+      if (const char* e = std::getenv("QFE_STREAMS")) ns = std::max(1, std::min(8, std::atoi(e)));
+      side.assign(ns - 1, nullptr);
+      side_ev.assign(ns - 1, nullptr);
+      for (auto& st : side) HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      for (auto& e : side_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    }
     setup_rys();
     std::vector<int> bf0(nsh), nf(nsh);
     for (int s = 0; s < nsh; ++s) {
@@ -1014,12 +1040,24 @@ struct Engine {
     unsigned long long* Kx = det && dK ? Kx_d.p : nullptr;
     double* Jd = det ? nullptr : dJ;
     nq = nq32 = 0;
+    // the class kernels over 1 + side.size() streams, each class to the
+    // stream with the least work so far (work ~ quartets x functions per
+    // quartet); profiled engines keep one stream
+    const int nst = profile ? 1 : 1 + (int)side.size();
+    std::vector<double> load(nst, 0.0);
+    if (nst > 1) {
+      HIP_CHECK(hipEventRecord(fork_ev, s));
+      for (int k = 1; k < nst; ++k) HIP_CHECK(hipStreamWaitEvent(side[k - 1], fork_ev, 0));
+    }
     for (int prec = 0; prec < 2; ++prec) {
       const long long* b = bval.data() + prec * (ncp + 1);
       for (int cp = 0; cp < ncp; ++cp) {
         const long long n = b[cp + 1] - b[cp];
         if (n <= 0) continue;
         (prec ? nq32 : nq) += n;
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[k] += (double)n * nfun_pair_class(cp_bra[cp]) * nfun_pair_class(cp_ket[cp]);
+        hipStream_t cs = k == 0 ? s : side[k - 1];
         JKLaunch a{tasks_p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jd, det ? nullptr : dK, nbf, kscale,
                    prec == 1, ppos32_d.p, pk32_d.p, &ra32,
                    segs_d.p + cp_seg_begin[cp], cp_seg_begin[cp + 1] - cp_seg_begin[cp],
@@ -1028,11 +1066,15 @@ struct Engine {
                    pmax_d.p, prim_screen ? 1 : 0, det ? 1 : 0, Jx, Kx, fxo_d.p, sh_bf0_h.data(), nsh,
                    kcol_tile, kets_d.p, shells_d.p};
         hipEvent_t* e = profile ? &cls_ev[4 * (size_t)cp + 2 * prec] : nullptr;
-        if (e) HIP_CHECK(hipEventRecord(e[0], s));
-        launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, s);
+        if (e) HIP_CHECK(hipEventRecord(e[0], cs));
+        launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, cs);
         HIP_CHECK(hipGetLastError());
-        if (e) HIP_CHECK(hipEventRecord(e[1], s));
+        if (e) HIP_CHECK(hipEventRecord(e[1], cs));
       }
+    }
+    for (int k = 1; k < nst; ++k) {  // join: everything after waits for every class
+      HIP_CHECK(hipEventRecord(side_ev[k - 1], side[k - 1]));
+      HIP_CHECK(hipStreamWaitEvent(s, side_ev[k - 1], 0));
     }
     if (jslots) {
       for (const JTileGroup& g : jt_groups) {

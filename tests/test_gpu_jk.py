@@ -241,3 +241,20 @@ def test_primitive_screening_changes_nothing_visible(monkeypatch):
           % (np.max(np.abs(J1 - J0)), np.max(np.abs(K1 - K0))), flush=True)
     assert np.max(np.abs(J1 - J0)) < 1e-12 and np.max(np.abs(K1 - K0)) < 1e-12
     assert np.max(np.abs(J1 - Jr)) < 1e-10 and np.max(np.abs(K1 - Kr)) < 1e-10
+
+
+@pytest.mark.parametrize("mix", [False, True])
+def test_concurrent_class_streams_match_one_stream(monkeypatch, mix):
+    """The class kernels spread over the engine's streams (the default) give
+    bit for bit the J and K of one stream (QFE_STREAMS=1): the deterministic
+    sums do not depend on the order in which the classes run."""
+    from qdk_chemistry_amd.basis import load_basis
+    from qdk_chemistry_amd.eri import ERIConfig, create
+    from qdk_chemistry_amd.molecule import water_cluster
+    mol = water_cluster(4, 13)
+    b = load_basis("cc-pvdz", mol.numbers, mol.coords)
+    P = random_density(b.nbf, 19, 0.3)
+    J1, K1 = create(b, mol, ERIConfig(is_mix_precision=mix)).build_JK(P)
+    monkeypatch.setenv("QFE_STREAMS", "1")
+    J0, K0 = create(b, mol, ERIConfig(is_mix_precision=mix)).build_JK(P)
+    assert np.array_equal(J1, J0) and np.array_equal(K1, K0)
