@@ -344,8 +344,11 @@ struct Engine {
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> side_ev;
   hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> done_ev;  // per (precision, class pair): its kernels are done
 
   ~Engine() {
+    for (auto& e : done_ev)
+      if (e) (void)hipEventDestroy(e);
     for (auto& e : side_ev)
       if (e) (void)hipEventDestroy(e);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
@@ -859,6 +862,10 @@ struct Engine {
       cls_ev.assign(4 * (size_t)ncp, nullptr);
       for (auto& e : cls_ev) HIP_CHECK(hipEventCreate(&e));
     }
+    if (!side.empty()) {
+      done_ev.assign(2 * (size_t)ncp, nullptr);
+      for (auto& e : done_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
   }
 
   unsigned char* scratch_ptr = nullptr;  // this device's no-alloc scratch (device address)
@@ -1040,23 +1047,38 @@ struct Engine {
     unsigned long long* Kx = det && dK ? Kx_d.p : nullptr;
     double* Jd = det ? nullptr : dJ;
     nq = nq32 = 0;
-    // the class kernels over 1 + side.size() streams, each class to the
-    // stream with the least work so far (work ~ quartets x functions per
-    // quartet); profiled engines keep one stream
+    // the class kernels over 1 + side.size() streams: the classes by work
+    // (quartets x functions per quartet), largest first, each to the stream
+    // with the least work so far; profiled engines keep one stream
     const int nst = profile ? 1 : 1 + (int)side.size();
-    std::vector<double> load(nst, 0.0);
-    if (nst > 1) {
-      HIP_CHECK(hipEventRecord(fork_ev, s));
-      for (int k = 1; k < nst; ++k) HIP_CHECK(hipStreamWaitEvent(side[k - 1], fork_ev, 0));
-    }
+    struct Job {
+      double cost;
+      int prec, cp;
+    };
+    std::vector<Job> jobs;
     for (int prec = 0; prec < 2; ++prec) {
       const long long* b = bval.data() + prec * (ncp + 1);
       for (int cp = 0; cp < ncp; ++cp) {
         const long long n = b[cp + 1] - b[cp];
         if (n <= 0) continue;
         (prec ? nq32 : nq) += n;
+        jobs.push_back({(double)n * nfun_pair_class(cp_bra[cp]) * nfun_pair_class(cp_ket[cp]), prec, cp});
+      }
+    }
+    if (nst > 1)
+      std::stable_sort(jobs.begin(), jobs.end(), [](const Job& x, const Job& y) { return x.cost > y.cost; });
+    std::vector<double> load(nst, 0.0);
+    if (nst > 1) {
+      HIP_CHECK(hipEventRecord(fork_ev, s));
+      for (int k = 1; k < nst; ++k) HIP_CHECK(hipStreamWaitEvent(side[k - 1], fork_ev, 0));
+    }
+    for (const Job& job : jobs) {
+      {
+        const int prec = job.prec, cp = job.cp;
+        const long long* b = bval.data() + prec * (ncp + 1);
+        const long long n = b[cp + 1] - b[cp];
         const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[k] += (double)n * nfun_pair_class(cp_bra[cp]) * nfun_pair_class(cp_ket[cp]);
+        load[k] += job.cost;
         hipStream_t cs = k == 0 ? s : side[k - 1];
         JKLaunch a{tasks_p + b[cp], n, pairs_d.p, ppos_d.p, pk_d.p, &ra, dP, Jd, det ? nullptr : dK, nbf, kscale,
                    prec == 1, ppos32_d.p, pk32_d.p, &ra32,
@@ -1070,18 +1092,30 @@ struct Engine {
         launch_jk(cp_bra[cp], cp_ket[cp], basis.pure, a, cs);
         HIP_CHECK(hipGetLastError());
         if (e) HIP_CHECK(hipEventRecord(e[1], cs));
+        if (nst > 1) HIP_CHECK(hipEventRecord(done_ev[(size_t)prec * ncp + cp], cs));
       }
     }
-    for (int k = 1; k < nst; ++k) {  // join: everything after waits for every class
-      HIP_CHECK(hipEventRecord(side_ev[k - 1], side[k - 1]));
-      HIP_CHECK(hipStreamWaitEvent(s, side_ev[k - 1], 0));
-    }
+    // J_cd sums per ket size: each group starts once the classes whose kets
+    // have that size are done (the other classes may still run)
     if (jslots) {
-      for (const JTileGroup& g : jt_groups) {
+      for (size_t gi = 0; gi < jt_groups.size(); ++gi) {
+        const JTileGroup& g = jt_groups[gi];
+        hipStream_t gs = s;
+        if (nst > 1) {
+          const int k = (int)(gi % nst);
+          gs = k == 0 ? s : side[k - 1];
+          for (const Job& job : jobs)
+            if (nfun_pair_class(cp_ket[job.cp]) == g.nk)
+              HIP_CHECK(hipStreamWaitEvent(gs, done_ev[(size_t)job.prec * ncp + job.cp], 0));
+        }
         launch_jcd_tiles(jtiles_d.p + g.begin, g.count, g.nk, g.nsb, nslot_d.p, jslot_d.p, mbase_d.p, jmask_p,
-                         jbuf_p, pairs_d.p, Jd, nbf, Jx, fxo_d.p, s);
+                         jbuf_p, pairs_d.p, Jd, nbf, Jx, fxo_d.p, gs);
         HIP_CHECK(hipGetLastError());
       }
+    }
+    for (int k = 1; k < nst; ++k) {  // join: everything after waits for every stream
+      HIP_CHECK(hipEventRecord(side_ev[k - 1], side[k - 1]));
+      HIP_CHECK(hipStreamWaitEvent(s, side_ev[k - 1], 0));
     }
   }
 
