@@ -392,6 +392,9 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the ERI gradient timing")
     ap.add_argument("--no-atomics", action="store_true", help="skip the use_atomics = True builds")
     ap.add_argument("--no-c5", action="store_true", help="skip the C20H42/cc-pVTZ FP64 and mixed SCF")
+    ap.add_argument("--profiled-only", action="store_true",
+                    help="time only the profiled engine (one stream; for profiler runs: one run of each class kernel "
+                         "per build)")
     ap.add_argument("--cpu-stride", type=int, default=96)
     ap.add_argument("--mo-waters", type=int, default=4, help="water cluster of the MO-transform timing (0: skip)")
     ap.add_argument("--mo-nt", type=int, default=48)
@@ -519,6 +522,19 @@ def main():
         log(f"ERI gradient: {grad['seconds'] * 1e3:.1f} ms ({gst['quartets']:.4g} quartets, "
             f"kernels {gst['ms_eri']:.1f} ms)")
     eng.close()
+    if args.profiled_only:
+        out = {"metric": METRIC, "value": s64["quartets_per_s"], "unit": "quartets/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": s64["ms_per_build"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": f"synthetic seeded geometry, {density} density",
+               "config": {"workload": f"RHF J/K build, {desc}, nbf={basis.nbf}, eri_threshold 1e-12",
+                          "path": "fp64 profiled engine",
+                          "fp64_profiled": {k: v for k, v in s64.items() if k != "dominant"}},
+               "roofline": {"bound": "fp64-valu", "kernel": "ERI class (%d %d|%d %d), FP64" % cls,
+                            "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                            "frac": achieved / FP64_PEAK, "flops_per_launch": f_launch, "ms_per_launch": ms_launch}}
+        print(json.dumps(out), flush=True)
+        return
     # The headline: the same builds on an engine without per-class events, so
     # that its class kernels run concurrently on the engine's streams (the
     # profiled engine above launches them one after another, for clean

@@ -159,6 +159,15 @@ QFE_DI double pload(const double* __restrict__ P, unsigned e) {
   return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(P) + (e << 3));
 }
 
+// The one-root classes stage the Boys rows they read (F_0 .. F_7 at every
+// grid point, 8 values a row) in LDS after the exchange and density rows:
+// their kernels wait on these per-lane gathers more than on anything else.
+constexpr int kLdsBoysW = 8;
+template <class C>
+constexpr bool lds_boys() {
+  return C::LA <= 2 && C::LB <= 2 && C::LC <= 2 && C::LD <= 2 && C::NR == 1;
+}
+
 template <class C, int Q0, int Q1, typename R, bool PROW, int K0, int K1, int BLK>
 __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ segs,
                                                     const long long* __restrict__ offs,
@@ -204,6 +213,13 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
   const double* Pa = PROW ? lds + (size_t)S::NR * krow : nullptr;
   __shared__ unsigned long long sjab[2 * NQ];  // J_ab of the block's bra functions
   for (int i = threadIdx.x; i < 2 * NQ; i += blockDim.x) sjab[i] = 0ull;
+  constexpr bool LBOYS = lds_boys<C>();
+  R* btab = nullptr;
+  if constexpr (LBOYS) {
+    btab = reinterpret_cast<R*>(lds + (size_t)S::NR * krow + (PROW ? (size_t)S::NR * W : 0));
+    for (int i = threadIdx.x; i < BOYS_NG * kLdsBoysW; i += blockDim.x)
+      btab[i] = ra.boys[(size_t)(i / kLdsBoysW) * BOYS_W + i % kLdsBoysW];
+  }
   for (int r = 0; r < S::NR; ++r) {
     const size_t ao = r < S::NMA ? (size_t)(i0 + S::MA0 + r) : (size_t)(j0 + S::MB_LO + (r - S::NMA));
     for (int c = threadIdx.x; c < (int)krow; c += blockDim.x) Ks[(size_t)r * krow + c] = 0.0;
@@ -248,7 +264,8 @@ __global__ __launch_bounds__(BLK) void k_eri_jk_seg(const int2* __restrict__ seg
     // the wave-uniform bra loop, except in classes with an f shell (their
     // long primitive bodies hide the per-lane break's wait, and the uniform
     // loop costs them registers)
-    contract_chunk<C, Q0, Q1, R, K0, K1, !FCLS>(bra, ket, ppos, pk, ra, acc, lnp, sig);
+    contract_chunk<C, Q0, Q1, R, K0, K1, !FCLS, LBOYS ? kLdsBoysW : BOYS_W>(bra, ket, ppos, pk, ra, acc, lnp,
+                                                                             sig, btab);
     const double deg = dbra * (ket.sa == ket.sb ? 1.0 : 2.0) * (tk.x == tk.y ? 1.0 : 2.0);
     const double degj = inC ? deg : 0.0;
     double Pkl[C::NKETS], jkl[C::NKETS];
@@ -447,9 +464,11 @@ struct ClassOps {
   // dynamic LDS bytes of a segment block over W exchange columns: the K rows
   // (doubles, or fixed-point pairs in deterministic builds) and the P rows
   // with prow; seg_total adds the static J_ab slots
+  // bytes of the Boys rows staged in LDS (one-root classes; sized for FP64)
+  static constexpr size_t BOYS_LDS = lds_boys<C>() ? (size_t)BOYS_NG * kLdsBoysW * sizeof(double) : 0;
   template <int Q0, int Q1>
   static size_t seg_bytes(int W, bool prow, bool det) {
-    return (size_t)SegRows<C, Q0, Q1>::NR * W * sizeof(double) * ((det ? 2 : 1) + (prow ? 1 : 0));
+    return (size_t)SegRows<C, Q0, Q1>::NR * W * sizeof(double) * ((det ? 2 : 1) + (prow ? 1 : 0)) + BOYS_LDS;
   }
   template <int Q0, int Q1>
   static size_t seg_total(int W, bool prow, bool det) {
@@ -458,8 +477,8 @@ struct ClassOps {
   // the widest W whose block fits lim bytes
   template <int Q0, int Q1>
   static int seg_width(size_t lim, bool prow, bool det) {
-    const size_t fixed = 2 * (size_t)(Q1 - Q0) * sizeof(double);
-    const size_t per = seg_bytes<Q0, Q1>(1, prow, det);
+    const size_t fixed = 2 * (size_t)(Q1 - Q0) * sizeof(double) + BOYS_LDS;
+    const size_t per = seg_bytes<Q0, Q1>(1, prow, det) - BOYS_LDS;
     return lim > fixed ? (int)std::min<size_t>((lim - fixed) / per, (size_t)1 << 30) : 0;
   }
 

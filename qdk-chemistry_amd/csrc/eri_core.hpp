@@ -156,12 +156,14 @@ QFE_DI float fexp(float x) { return expf(x); }
 // downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) fills the rest.  Above:
 // F_0 from the asymptotic form and the upward recursion (stable for T > M;
 // e^-T < 4e-18 is dropped).
-template <int M, typename R>
+// tab: rows of BW values F_m(k * BOYS_DT), m < BW
+template <int M, typename R, int BW = BOYS_W>
 QFE_DI void boys_fm(R T, const R* __restrict__ tab, R (&F)[M + 1]) {
+  static_assert(M + 7 <= BW, "the Taylor series reads F_M .. F_{M+6}");
   if (T < R(BOYS_TSW)) {
     const int g = (int)(T * R(1.0 / BOYS_DT) + R(0.5));
     const R d = (R)g * R(BOYS_DT) - T;
-    const R* __restrict__ row = tab + (size_t)g * BOYS_W;
+    const R* __restrict__ row = tab + (size_t)g * BW;
     if constexpr (M <= 2) {
       sfor<M + 1>([&](auto m) {
         R s = row[m + 6];
@@ -191,12 +193,13 @@ QFE_DI void boys_fm(R T, const R* __restrict__ tab, R (&F)[M + 1]) {
   }
 }
 
-template <int N, typename R>
-QFE_DI void rys_roots(R T, const RysArgsT<R>& ra, R (&u)[N], R (&w)[N]) {
+// boys: the Boys table (rows of BW values) if not ra.boys
+template <int N, typename R, int BW = BOYS_W>
+QFE_DI void rys_roots(R T, const RysArgsT<R>& ra, R (&u)[N], R (&w)[N], const R* boys = nullptr) {
   if constexpr (N <= 2) {
     // Gauss rule of the Boys measure from its moments F_0..F_{2N-1}
     R F[2 * N];
-    boys_fm<2 * N - 1>(T, ra.boys, F);
+    boys_fm<2 * N - 1, R, BW>(T, BW == BOYS_W ? ra.boys : boys, F);
     if constexpr (N == 1) {
       w[0] = F[0];
       u[0] = F[1] * frcp(F[0]);
@@ -365,11 +368,14 @@ QFE_DI R prim_ln_cut(const double* pmax, int prim_screen) {
 // readfirstlane and the restrict pointer the bra data are scalar loads; a
 // lane past its cut runs on with a zero prefactor (in SIMT a masked lane
 // would cost the same).
-template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC, bool UNIFORM = false>
+// BW < BOYS_W: the one- and two-root classes read the Boys rows from boys
+// (rows of BW values, e.g. staged in LDS) instead of ra.boys.
+template <class C, int Q0, int Q1, typename R, int K0 = 0, int K1 = C::NKETC, bool UNIFORM = false,
+          int BW = BOYS_W>
 QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
                            const typename Vec4<R>::type* __restrict__ ppos, const R* __restrict__ pk,
                            const RysArgsT<R>& ra, R (&acc)[Q1 - Q0][K1 - K0], R lnp = R(-3.0e38),
-                           const R* __restrict__ sigp = nullptr) {
+                           const R* __restrict__ sigp = nullptr, const R* boys = nullptr) {
   constexpr int NK = K1 - K0;
   const R TWO_PI_52 = R(34.98683665524972);  // 2 pi^(5/2)
   sfor<Q1 - Q0>([&](auto q) { sfor<NK>([&](auto k) { acc[q][k] = R(0); }); });
@@ -411,7 +417,7 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
       const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
       const R pref = TWO_PI_52 * Kb * Kk * rs;
       R u[C::NR], w[C::NR];
-      rys_roots<C::NR>(T, ra, u, w);
+      rys_roots<C::NR, R, BW>(T, ra, u, w, boys);
       const R qr = q * rpq, pr = p * rpq;
       const R h2p = ra.h2[ib];
 #pragma unroll 1
