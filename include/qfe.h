@@ -63,7 +63,8 @@ typedef struct {
   int mix_precision;           /* 1: FP32 for quartets whose Schwarz bound < mix_threshold */
   double mix_threshold;        /* 1e-6 */
   int verbose;
-  int profile;                 /* 1: time every class kernel group with HIP events */
+  int profile;                 /* 1: time every class kernel group with HIP events (its class kernels then
+                                  run one after another; else they spread over QFE_STREAMS streams) */
   int use_atomics;             /* 0 (the reference's default, core/scf.h:81-82): bitwise-repeatable
                                   J/K (order-independent fixed-point accumulation); 1: FP64 atomics */
 } qfe_config;
@@ -227,10 +228,13 @@ int qfe_get_stats(const qfe_handle* h, qfe_stats* out);
  * cls[4i..4i+3] = (la, lb, lc, ld) as evaluated, quartets[i] and
  * prim_quartets[i] the screened shell quartets and their primitive quartets
  * over significant primitive pairs, exec_prim_quartets[i] (may be NULL) the
- * primitive quartets the kernels evaluated after primitive screening, and
+ * primitive quartets the kernels evaluated after primitive screening
+ * (counted on this call by a counting kernel over the last J/K build's task
+ * list, synchronously on the engine's stream; 0 after a gradient call), and
  * ms[i] the kernel time of that class (the last two only with
- * qfe_config.profile, else 0).  Returns n (number of class pairs), or -1 for
- * a null handle.  No reference counterpart: instrumentation. */
+ * qfe_config.profile, else 0; a profiled engine launches its class kernels
+ * one after another).  Returns n (number of class pairs), or -1 for a null
+ * handle or a failed count.  No reference counterpart: instrumentation. */
 int qfe_class_stats(const qfe_handle* h, int precision, int cap, int* cls, double* quartets,
                     double* prim_quartets, double* exec_prim_quartets, double* ms);
 /* Chunk geometry of the J/K kernels of class (la lb | lc ld) as compiled:
