@@ -457,9 +457,12 @@ struct ClassOps {
   // all of them, so light classes use large blocks to keep enough waves per
   // CU resident (the register budget per thread shrinks with the block:
   // 128 VGPRs at 1024 threads, 170 at 768, 256 at 512); 768 for the L = 2
-  // classes and the L = 3 classes with an (ss) ket, which fit 170
+  // classes and the L = 3 classes with an (ss) ket, which fit 170; 512 for
+  // the heavier classes with an (ss) or (ps) ket, which fit 256
   static constexpr int SEG_BLK =
-      K::F ? 256 : (C::L <= 1 ? 1024 : (C::L <= 3 ? ((C::L == 2 || C::FC == 0) ? 768 : 512) : 256));
+      K::F ? 256
+           : (C::L <= 1 ? 1024
+                        : (C::L <= 3 ? ((C::L == 2 || C::FC == 0) ? 768 : 512) : (C::FC <= 1 ? 512 : 256)));
   // doubles of the Schwarz staging block per pair (classes with f shells)
   static constexpr int SCHWARZ_BLOCK = K::F ? C::NBRA * C::NKETC : 0;
 
