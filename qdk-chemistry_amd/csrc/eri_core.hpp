@@ -147,6 +147,22 @@ QFE_DI float frsq(float x) {
   return 1.0f / sqrtf(x);
 #endif
 }
+// An empty asm that reads x: x must be in a register at this point, so the
+// compiler issues its load before it and cannot sink the load below a later
+// branch.  held_s for wave-uniform values (scalar registers), held_v for
+// per-lane values.
+template <typename T>
+QFE_DI void held_s(const T& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" ::"s"(x));
+#endif
+}
+template <typename T>
+QFE_DI void held_v(const T& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" ::"v"(x));
+#endif
+}
 QFE_DI double fexp(double x) { return exp(x); }
 QFE_DI float fexp(float x) { return expf(x); }
 
@@ -387,27 +403,43 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
   const R sb0 = sig[bra.prim0];  // the bra's largest factor
   // ket primitives outer (per-lane data), bra primitives inner: the bra pair
   // is usually shared by the whole wave, so the inner loads are broadcasts
+  // with UNIFORM, each iteration holds its primitive's data in registers
+  // before the cut test, so that the loads share one round trip instead of
+  // following each other across the test (a primitive past the cut still
+  // lies in its pair's range: the reads stay in bounds)
+  using V4 = typename Vec4<R>::type;
   for (int ik = ket.prim0; ik < ket.prim0 + ket.nprim; ++ik) {
     const R sk = sig[ik];
+    V4 Q;
+    R Kk, h2q;  // h2q: used from a ket (pp) or (ds) on
+    if constexpr (UNIFORM) {
+      Q = ppos[ik], Kk = pk[ik], h2q = ra.h2[ik];
+      held_v(Q.x), held_v(Q.y), held_v(Q.z), held_v(Q.w), held_v(Kk);
+      if constexpr (C::FC >= 2) held_v(h2q);
+    }
     if (sk + sb0 < lnp) break;  // this and every later ket primitive
+    if constexpr (!UNIFORM) Q = ppos[ik], Kk = pk[ik], h2q = ra.h2[ik];
     const R blim = lnp - sk;
-    const auto Q = ppos[ik];
-    const R Kk = pk[ik];
     const R q = Q.x;
     const R QCx = Q.y - Cx, QCy = Q.z - Cy, QCz = Q.w - Cz;
-    const R h2q = ra.h2[ik];
     for (int ib0 = bra.prim0; ib0 < bra.prim0 + bra.nprim; ++ib0) {
       int ib = ib0;
       bool on = true;
+      V4 P;
+      R Kb, h2p;  // h2p: used from a bra (pp) or (ds) on
       if constexpr (UNIFORM) {
         ib = __builtin_amdgcn_readfirstlane(ib0);
-        on = !(sig[ib] < blim);
+        const R sgb = sig[ib];
+        P = ppos[ib], Kb = pk[ib], h2p = ra.h2[ib];
+        held_s(P.x), held_s(P.y), held_s(P.z), held_s(P.w), held_s(Kb);
+        if constexpr (C::EA >= 2) held_s(h2p);
+        on = !(sgb < blim);
         if (__ballot(on) == 0ull) break;  // past the cut on every lane
+        Kb = on ? Kb : R(0);
       } else {
         if (sig[ib] < blim) break;  // this and every later bra primitive
+        P = ppos[ib], Kb = pk[ib];
       }
-      const auto P = ppos[ib];
-      const R Kb = on ? pk[ib] : R(0);
       const R p = P.x;
       const R PAx = P.y - Ax, PAy = P.z - Ay, PAz = P.w - Az;
       const R PQx = P.y - Q.y, PQy = P.z - Q.z, PQz = P.w - Q.w;
@@ -419,7 +451,7 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
       R u[C::NR], w[C::NR];
       rys_roots<C::NR, R, BW>(T, ra, u, w, boys);
       const R qr = q * rpq, pr = p * rpq;
-      const R h2p = ra.h2[ib];
+      if constexpr (!UNIFORM) h2p = ra.h2[ib];
 #pragma unroll 1
       for (int r = 0; r < C::NR; ++r) {
         const R ur = u[r];
