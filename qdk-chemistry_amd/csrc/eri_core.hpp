@@ -203,8 +203,9 @@ QFE_DI void boys_fm(R T, const R* __restrict__ tab, R (&F)[M + 1]) {
       });
     }
   } else {
-    const R it = frcp(T);
-    F[0] = R(0.88622692545275801365) * frsq(T);  // sqrt(pi)/2 / sqrt(T)
+    const R is = frsq(T);
+    const R it = is * is;                     // 1/T from 1/sqrt(T): no second reciprocal
+    F[0] = R(0.88622692545275801365) * is;  // sqrt(pi)/2 / sqrt(T)
     sfor<M>([&](auto m) { F[m + 1] = R(2 * m + 1) * F[m] * (R(0.5) * it); });
   }
 }
@@ -448,6 +449,17 @@ QFE_DI void contract_chunk(const PairHdr& bra, const PairHdr& ket,
       const R rho = p * q * rpq;
       const R T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
       const R pref = TWO_PI_52 * Kb * Kk * rs;
+      if constexpr (C::L == 1 && C::LA == 1) {
+        // (ps|ss): one root, and the integral is linear in it; with the
+        // weight w = F_0 and w u = F_1 no division by F_0 is needed:
+        // (p_i s|ss) = pref (F_0 PA_i - q / (p + q) F_1 PQ_i)
+        R F[2];
+        boys_fm<1, R, BW>(T, BW == BOYS_W ? ra.boys : boys, F);
+        const R a0 = pref * F[0], a1 = pref * F[1] * (q * rpq);
+        const R PA[3] = {PAx, PAy, PAz}, PQ[3] = {PQx, PQy, PQz};
+        sfor<Q1 - Q0>([&](auto qq) { acc[qq][0] += fma(a0, PA[Q0 + qq], -a1 * PQ[Q0 + qq]); });
+        continue;
+      }
       R u[C::NR], w[C::NR];
       rys_roots<C::NR, R, BW>(T, ra, u, w, boys);
       const R qr = q * rpq, pr = p * rpq;

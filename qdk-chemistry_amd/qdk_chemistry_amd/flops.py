@@ -17,7 +17,8 @@ f_prim, per primitive quartet of one chunk:
             1/(2p) and 1/(2q) are table loads (no flops).
   roots     the cheapest branch of rys_roots (large-T Boys moments for one or
             two roots, the large-T Hermite limit for three or more), so the
-            count is a lower bound of what executes.
+            count is a lower bound of what executes.  (ps|ss) has no root
+            and no per-root terms: see _ps_ss.
   per root  B00, B10, B01, the two root scalings (10), C00 and C00' in x, y,
             z (12), the weight on Iz (1) = 23; the 2-D recursion and the two
             1-D transfers in each direction (rys_1d_x); the chunk's assembly,
@@ -91,8 +92,20 @@ def _roots(nr, L=1):
             return 9                # F0 only (rsqrt with Newton steps, mul): no root is needed
         return 10 + 2 + 10          # large-T F0, F1 (rcp + rsqrt with Newton steps, mul) + u = F1 / F0
     if nr == 2:
-        return 10 + 10 + 6 + 30 + 10 + 14  # F0..F3 (large T) + the 2 x 2 moment solve
+        # F0..F3 (large T: rsqrt with Newton steps, 1/T as its square, F0
+        # and the upward steps) + the 2 x 2 moment solve
+        return 10 + 1 + 6 + 30 + 10 + 14
     return 20 + 2 * nr               # large-T Hermite limit: rcp, rsqrt, 2n mul
+
+
+def _ps_ss(nq):
+    """f_prim of one (ps|ss) chunk of nq bra functions.  The kernel skips the
+    root: with one root the integral is linear in it, and w = F_0, w u = F_1
+    give (p_i s|ss) = pref (F_0 PA_i - q/(p+q) F_1 PQ_i).  Set-up (30), the
+    large-T F_0, F_1 (rsqrt with Newton steps 10, 1/T as its square 1,
+    F_0 1, F_1 2), pref F_0 and pref F_1 q/(p+q) (3), and per bra function
+    1 mul, 1 FMA and the accumulator add (4)."""
+    return _setup(1, 0) + 14 + 3 + 4 * nq
 
 
 SETUP = 3 + 3 + 1 + 10 + 1 + 2 + 6 + 3 + 2   # 31 with every term used
@@ -135,7 +148,10 @@ def class_flops(la, lb, lc, ld, pure=True):
                        if abs(Sa[ma, a] * Sb[mb, b] - 1.0) < 1e-15)
             asm += nk * (nnz * 2 + ones * 1 + (nnz - ones) * 2 + 1)
         EA, FC = la + lb, lc + ld
-        f_prim += _setup(EA, FC) + _roots(nr, L) + nr * (_per_root(EA, FC) + onedim + asm)
+        if L == 1 and la == 1:
+            f_prim += _ps_ss(q1 - q0)
+        else:
+            f_prim += _setup(EA, FC) + _roots(nr, L) + nr * (_per_root(EA, FC) + onedim + asm)
         # ket transform of this chunk's cartesians into every ket function
         nnz_ket = sum(1 for mc in range(nsc) for md in range(nsd) for k in range(k0, k1)
                       if Sc[mc, k // ncd] * Sd[md, k % ncd] != 0.0)
