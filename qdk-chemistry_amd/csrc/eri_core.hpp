@@ -166,34 +166,37 @@ QFE_DI void held_v(const T& x) {
 QFE_DI double fexp(double x) { return exp(x); }
 QFE_DI float fexp(float x) { return expf(x); }
 
-// F_0..F_M at T.  Below BOYS_TSW: 7-term Taylor expansion about the nearest
-// grid point (|dT| <= 0.025, truncation < 2e-16 relative); for M <= 2 every
-// order is expanded directly (no exp), otherwise F_M is expanded and the
-// downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) fills the rest.  Above:
-// F_0 from the asymptotic form and the upward recursion (stable for T > M;
-// e^-T < 4e-18 is dropped).
+// F_0..F_M at T.  Below BOYS_TSW: Taylor expansion about the nearest grid
+// point, |dT| <= 0.025, with NT terms: 7 in double precision (truncation
+// < 2e-16 relative), 4 in single precision (the first dropped term is below
+// F_{m+4} dT^4 / 4! <= 1.7e-8 F_m, under the FP32 rounding unit); for M <= 2
+// every order is expanded directly (no exp), otherwise F_M is expanded and
+// the downward recursion F_m = (2T F_{m+1} + e^-T)/(2m+1) fills the rest.
+// Above: F_0 from the asymptotic form and the upward recursion (stable for
+// T > M; e^-T < 4e-18 is dropped).
 // tab: rows of BW values F_m(k * BOYS_DT), m < BW
 template <int M, typename R, int BW = BOYS_W>
 QFE_DI void boys_fm(R T, const R* __restrict__ tab, R (&F)[M + 1]) {
-  static_assert(M + 7 <= BW, "the Taylor series reads F_M .. F_{M+6}");
+  constexpr int NT = sizeof(R) == 4 ? 4 : 7;
+  static_assert(M + 7 <= BW, "the Taylor series reads at most F_M .. F_{M+6}");
   if (T < R(BOYS_TSW)) {
     const int g = (int)(T * R(1.0 / BOYS_DT) + R(0.5));
     const R d = (R)g * R(BOYS_DT) - T;
     const R* __restrict__ row = tab + (size_t)g * BW;
     if constexpr (M <= 2) {
       sfor<M + 1>([&](auto m) {
-        R s = row[m + 6];
-        sfor<6>([&](auto kk) {
-          constexpr int k = 5 - kk;
+        R s = row[m + NT - 1];
+        sfor<NT - 1>([&](auto kk) {
+          constexpr int k = NT - 2 - kk;
           s = fma(s, d * R(1.0 / (double)(k + 1)), row[m + k]);
         });
         F[m] = s;
       });
     } else {
       const R e = fexp(-T);
-      R s = row[M + 6];
-      sfor<6>([&](auto kk) {
-        constexpr int k = 5 - kk;
+      R s = row[M + NT - 1];
+      sfor<NT - 1>([&](auto kk) {
+        constexpr int k = NT - 2 - kk;
         s = fma(s, d * R(1.0 / (double)(k + 1)), row[M + k]);
       });
       F[M] = s;
