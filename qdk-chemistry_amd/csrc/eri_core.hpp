@@ -130,19 +130,18 @@ QFE_DI double frsq(double x) {
   return 1.0 / sqrt(x);
 #endif
 }
-// single precision: one Newton step on the hardware estimate
+// single precision: the hardware results (v_rcp_f32, v_rsq_f32: 1 ulp for
+// normal arguments, the FP32 rounding level), with no Newton step
 QFE_DI float frcp(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const float r = __builtin_amdgcn_rcpf(x);
-  return fmaf(fmaf(-x, r, 1.0f), r, r);
+  return __builtin_amdgcn_rcpf(x);
 #else
   return 1.0f / x;
 #endif
 }
 QFE_DI float frsq(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  const float r = __builtin_amdgcn_rsqf(x);
-  return r * fmaf(-0.5f * x * r, r, 1.5f);
+  return __builtin_amdgcn_rsqf(x);
 #else
   return 1.0f / sqrtf(x);
 #endif
